@@ -1,0 +1,551 @@
+"""bitwise_spgemm_amd — MI355X (gfx950) bit-wise sparse GEMM.
+
+Python host mirror of the C ABI in ``include/bws.h`` (``libbws.so``).  Device
+memory and streams come from PyTorch-ROCm; every call enqueues HIP kernels of
+``libbws.so`` on the current torch stream.  There is no fallback: if the
+library is not built, or no GPU is present, calls raise.
+
+Reference entry points and their counterparts here (ZSL98/bitwise_spgemm):
+
+* ``dense2bitmask`` (source/src/main.cu:591-628)          -> :func:`pack_bitmap`
+* ``layout_trans`` / ``uint8_layout_trans``
+  (backup/layout_trans.cu:75-97)                         -> :func:`unpack_bitplanes_f32`,
+                                                            :func:`unpack_bitplanes_u8`
+* ``csr2tiledcsr`` prefix + values (source/src/main.cu:499-588)
+                                                         -> :func:`row_ptr`, :func:`pack_values`
+* ``generate_groups_large`` (source/src/main.cu:50-247)   -> :func:`group_rows`
+* ``generate_spilled_csr`` (source/src/main.cu:445-496)   -> :func:`group_spilled_csr`
+* ``group2dense`` / ``dense2group_from_idx``
+  (source/src/main.cu:829-886)                           -> :func:`group_scatter`, :func:`group_gather`
+* ``spgemm_compute_1dthread_tcore_v3``
+  (source/src/transform.cuh:2062-2248), ``uint8_spmm_2d_1d``
+  (backup/layout_trans.cu:158-171)                       -> :func:`spmm`, :func:`spmm_groups`
+
+The reference-named aliases at the bottom of this module call the same code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbws.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "bws.h")
+
+# status codes (include/bws.h)
+BWS_OK = 0
+BWS_EINVAL = -1
+BWS_ECAPACITY = -2
+BWS_EUNSUPPORTED = -3
+BWS_EHIP = -4
+BWS_EWORKSPACE = -5
+BWS_ENODEVICE = -6
+
+DTYPE = {torch.float32: 0, torch.bfloat16: 1, torch.float16: 2}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x2": 3}
+
+
+class BwsError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        msg = lib().bws_status_string(status).decode()
+        if status == BWS_EHIP:
+            msg += f" (hipError {lib().bws_last_hip_error()})"
+        super().__init__(f"{where}: {msg}")
+
+
+class _Bitmap(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("ldw", C.c_int64),
+                ("words", C.c_void_p), ("row_ptr", C.c_void_p), ("values", C.c_void_p)]
+
+
+class _Groups(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("group_id", "group_mask", "group_vals", "member_row",
+                                          "spill_cnt", "spill_nnz", "spill_table", "spill_rev",
+                                          "grouped_nnz")]
+
+
+class _Plan(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("rows_per_wg", C.c_int32), ("n_per_wg", C.c_int32),
+                ("mfma_min_bits", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class _Problem(C.Structure):
+    _fields_ = [("A", _Bitmap), ("B", C.c_void_p), ("ldb", C.c_int64), ("N", C.c_int64),
+                ("C", C.c_void_p), ("ldc", C.c_int64)]
+
+
+_P, _I64, _I32, _U64, _F, _D, _SZ = (C.c_void_p, C.c_int64, C.c_int32, C.c_uint64, C.c_float,
+                                     C.c_double, C.c_size_t)
+_BP = C.POINTER(_Bitmap)
+
+_SIGS = {
+    "bws_status_string": ([_I32], C.c_char_p),
+    "bws_last_hip_error": ([], _I32),
+    "bws_version": ([], _I32),
+    "bws_pack_bitmap_f32": ([_P, _I64, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_unpack_bitmap_f32": ([_P, _I64, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_unpack_bitplanes_f32": ([_P, _I64, _I32, _P, _P], _I32),
+    "bws_unpack_bitplanes_u8": ([_P, _I64, _P, _P], _I32),
+    "bws_pack_bitplanes_i8": ([_P, _I64, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_bitplane_weights": ([_P], _I32),
+    "bws_bitplane_row_counts": ([_P, _I32, _I64, _I64, _I64, _P, _P], _I32),
+    "bws_row_ptr_workspace_size": ([_I64], _SZ),
+    "bws_row_ptr": ([_P, _I64, _I64, _I64, _P, _P, _SZ, _P], _I32),
+    "bws_pack_values_f32": ([_P, _I64, _I64, _I64, _P, _I64, _P, _P, _P], _I32),
+    "bws_transpose_bitmap": ([_P, _I64, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_transpose_workspace_size": ([_I64, _I64], _SZ),
+    "bws_transpose_values": ([_BP, _P, _I64, _P, _P, _P, _SZ, _P], _I32),
+    "bws_group_tiles": ([_I64, _I64, _I32], _I64),
+    "bws_group_rows": ([_BP, _I32, _I32, C.POINTER(_Groups), _P], _I32),
+    "bws_group_spilled_workspace_size": ([_I64, _I64, _I32], _SZ),
+    "bws_group_spilled_csr": ([_BP, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P], _I32),
+    "bws_group_scatter": ([_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _I64, _P], _I32),
+    "bws_group_gather": ([_P, _I64, _P, _I64, _I64, _I64, _P, _I32, _I32, _P, _P], _I32),
+    "bws_spmm_workspace_size": ([_BP, _I64, _I32, C.POINTER(_Plan)], _SZ),
+    "bws_spmm": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P, _SZ,
+                  _P], _I32),
+    "bws_spmm_groups": ([_BP, _P, _P, _I32, _I32, _P, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_spmm_tA": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P,
+                     _SZ, _P], _I32),
+    "bws_sddmm": ([_BP, _P, _P, _I32, _I64, _I64, _I64, _P, _I32, _P], _I32),
+    "bws_spmm_batched_workspace_size": ([_I32], _SZ),
+    "bws_spmm_batched": ([C.POINTER(_Problem), _I32, _I32, _I32, C.POINTER(_Plan), _P, _SZ, _P],
+                         _I32),
+    "bws_gen_bitmap_iid": ([_U64, _I64, _I64, _D, _P, _I64], _I32),
+    "bws_gen_bitmap_clustered": ([_U64, _I64, _I64, _I32, _D, _D, _P, _I64], _I32),
+    "bws_gen_f32": ([_U64, _I64, _I32, _D, _D, _P], _I32),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libbws.so (ctypes), with argument types set."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m bitwise_spgemm_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, ret) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ret
+        _lib = L
+    return _lib
+
+
+def header_functions():
+    """Names of every function declared in include/bws.h."""
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bws_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(status: int, where: str):
+    if status != BWS_OK:
+        raise BwsError(status, where)
+
+
+def _stream(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _require_gpu(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise ValueError("bitwise_spgemm_amd operates on GPU tensors (device='cuda')")
+
+
+def words_for(cols: int) -> int:
+    return (cols + 31) // 32
+
+
+# ----------------------------------------------------------------------------
+# bitmap container
+# ----------------------------------------------------------------------------
+@dataclass
+class Bitmap:
+    """A packed MSB-first bitmap [rows x ldw] (int32 storage of uint32 words),
+    optionally value-carrying (row_ptr [rows+1] int64 + values [nnz] f32)."""
+
+    words: torch.Tensor
+    rows: int
+    cols: int
+    row_ptr: Optional[torch.Tensor] = None
+    values: Optional[torch.Tensor] = None
+
+    @property
+    def ldw(self) -> int:
+        return self.words.shape[1] if self.words.dim() == 2 else words_for(self.cols)
+
+    def struct(self) -> _Bitmap:
+        return _Bitmap(self.rows, self.cols, self.ldw, self.words.data_ptr() if self.words.numel() else None,
+                       None if self.row_ptr is None else self.row_ptr.data_ptr(),
+                       None if self.values is None else self.values.data_ptr())
+
+    @property
+    def nnz(self) -> int:
+        if self.row_ptr is not None:
+            return int(self.row_ptr[-1].item())
+        return int(popcount(self.words).sum().item())
+
+    def with_values(self, values: Optional[torch.Tensor], row_ptr_t: Optional[torch.Tensor] = None):
+        rp = row_ptr_t if row_ptr_t is not None else (self.row_ptr if self.row_ptr is not None
+                                                      else row_ptr(self))
+        return Bitmap(self.words, self.rows, self.cols, rp, values)
+
+
+def popcount(words: torch.Tensor) -> torch.Tensor:
+    """Per-word popcount of int32-stored uint32 words (torch ops, for host checks)."""
+    x = words.to(torch.int64) & 0xFFFFFFFF
+    n = torch.zeros_like(x)
+    for _ in range(32):
+        n += x & 1
+        x >>= 1
+    return n
+
+
+def from_host_words(W, rows: int, cols: int, device="cuda") -> Bitmap:
+    """Upload a numpy uint32 [rows x ldw] bitmap."""
+    import numpy as np
+    W = np.ascontiguousarray(W, dtype=np.uint32)
+    return Bitmap(torch.from_numpy(W.view(np.int32)).to(device), rows, cols)
+
+
+def to_host_words(bm_or_words):
+    import numpy as np
+    w = bm_or_words.words if isinstance(bm_or_words, Bitmap) else bm_or_words
+    return w.cpu().numpy().view(np.uint32)
+
+
+# ----------------------------------------------------------------------------
+# (1) layout transforms
+# ----------------------------------------------------------------------------
+def pack_bitmap(D: torch.Tensor, ldw: Optional[int] = None) -> Bitmap:
+    """dense2bitmask: bit (31-i) of word w set iff D[r, 32w+i] != 0."""
+    _require_gpu(D)
+    assert D.dtype == torch.float32 and D.dim() == 2 and (D.stride(1) == 1 or D.shape[1] <= 1)
+    rows, cols = D.shape
+    ldw = ldw or words_for(cols)
+    W = torch.empty((rows, ldw), dtype=torch.int32, device=D.device)
+    _check(lib().bws_pack_bitmap_f32(_ptr(D), rows, cols, max(D.stride(0), cols, 1), _ptr(W), ldw,
+                                     _stream(D.device)), "bws_pack_bitmap_f32")
+    return Bitmap(W, rows, cols)
+
+
+def unpack_bitmap(A: Bitmap) -> torch.Tensor:
+    D = torch.empty((A.rows, A.cols), dtype=torch.float32, device=A.words.device)
+    _check(lib().bws_unpack_bitmap_f32(_ptr(A.words), A.rows, A.cols, A.ldw, _ptr(D), A.cols,
+                                       _stream(D.device)), "bws_unpack_bitmap_f32")
+    return D
+
+
+def unpack_bitplanes_f32(src: torch.Tensor, bit_width: int = 32) -> torch.Tensor:
+    """layout_trans: [bit_width, n] int32 planes of the raw FP32 bit pattern."""
+    _require_gpu(src)
+    src = src.contiguous().view(-1)
+    dst = torch.empty((bit_width, src.numel()), dtype=torch.int32, device=src.device)
+    _check(lib().bws_unpack_bitplanes_f32(_ptr(src), src.numel(), bit_width, _ptr(dst),
+                                          _stream(src.device)), "bws_unpack_bitplanes_f32")
+    return dst
+
+
+def unpack_bitplanes_u8(src: torch.Tensor) -> torch.Tensor:
+    """uint8_layout_trans: [8, n] int32 planes."""
+    _require_gpu(src)
+    src = src.contiguous().view(-1)
+    dst = torch.empty((8, src.numel()), dtype=torch.int32, device=src.device)
+    _check(lib().bws_unpack_bitplanes_u8(_ptr(src), src.numel(), _ptr(dst), _stream(src.device)),
+           "bws_unpack_bitplanes_u8")
+    return dst
+
+
+def pack_bitplanes_i8(D: torch.Tensor, ldw: Optional[int] = None) -> torch.Tensor:
+    """8 packed planes [8, rows, ldw] of an int8 matrix."""
+    _require_gpu(D)
+    assert D.dtype == torch.int8 and D.dim() == 2 and (D.stride(1) == 1 or D.shape[1] <= 1)
+    rows, cols = D.shape
+    ldw = ldw or words_for(cols)
+    P = torch.empty((8, rows, ldw), dtype=torch.int32, device=D.device)
+    _check(lib().bws_pack_bitplanes_i8(_ptr(D), rows, cols, D.stride(0), _ptr(P), ldw,
+                                       _stream(D.device)), "bws_pack_bitplanes_i8")
+    return P
+
+
+def bitplane_weights() -> list:
+    w = (C.c_float * 8)()
+    _check(lib().bws_bitplane_weights(C.cast(w, C.c_void_p)), "bws_bitplane_weights")
+    return list(w)
+
+
+def bitplane_row_counts(P: torch.Tensor, cols: int) -> torch.Tensor:
+    planes, rows, ldw = P.shape
+    out = torch.empty((planes, rows), dtype=torch.int32, device=P.device)
+    _check(lib().bws_bitplane_row_counts(_ptr(P), planes, rows, cols, ldw, _ptr(out),
+                                         _stream(P.device)), "bws_bitplane_row_counts")
+    return out
+
+
+def _workspace(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def row_ptr(A: Bitmap) -> torch.Tensor:
+    """Exclusive prefix of per-row set-bit counts, [rows+1] int64."""
+    dev = A.words.device
+    rp = torch.empty(A.rows + 1, dtype=torch.int64, device=dev)
+    ws = _workspace(lib().bws_row_ptr_workspace_size(A.rows), dev)
+    _check(lib().bws_row_ptr(_ptr(A.words), A.rows, A.cols, A.ldw, _ptr(rp), _ptr(ws), ws.numel(),
+                             _stream(dev)), "bws_row_ptr")
+    return rp
+
+
+def pack_values(D: torch.Tensor, A: Bitmap) -> Bitmap:
+    """Value-carrying bitmap of D's nonzeros (values in row, column order)."""
+    _require_gpu(D)
+    rp = A.row_ptr if A.row_ptr is not None else row_ptr(A)
+    nnz = int(rp[-1].item())
+    vals = torch.empty(max(nnz, 1), dtype=torch.float32, device=D.device)[:nnz]
+    _check(lib().bws_pack_values_f32(_ptr(D), A.rows, A.cols, D.stride(0), _ptr(A.words), A.ldw,
+                                     _ptr(rp), _ptr(vals) if nnz else _ptr(torch.empty(1, device=D.device)),
+                                     _stream(D.device)), "bws_pack_values_f32")
+    return Bitmap(A.words, A.rows, A.cols, rp, vals)
+
+
+def transpose(A: Bitmap, with_values: bool = True) -> Bitmap:
+    """Aᵀ as a bitmap (cols x rows); value-carrying if A is."""
+    dev = A.words.device
+    ldwt = words_for(A.rows)
+    WT = torch.empty((A.cols, ldwt), dtype=torch.int32, device=dev)
+    if A.row_ptr is None and (A.values is None or not with_values):
+        _check(lib().bws_transpose_bitmap(_ptr(A.words), A.rows, A.cols, A.ldw, _ptr(WT), ldwt,
+                                          _stream(dev)), "bws_transpose_bitmap")
+        return Bitmap(WT, A.cols, A.rows)
+    nnz = int(A.row_ptr[-1].item())
+    rpt = torch.empty(A.cols + 1, dtype=torch.int64, device=dev)
+    vt = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev) if A.values is not None else None
+    ws = _workspace(lib().bws_transpose_workspace_size(A.rows, A.cols), dev)
+    s = A.struct()
+    _check(lib().bws_transpose_values(C.byref(s), _ptr(WT), ldwt, _ptr(rpt), _ptr(vt), _ptr(ws),
+                                      ws.numel(), _stream(dev)), "bws_transpose_values")
+    return Bitmap(WT, A.cols, A.rows, rpt, None if vt is None else vt[:nnz])
+
+
+# ----------------------------------------------------------------------------
+# (2) row grouping
+# ----------------------------------------------------------------------------
+def group_tiles(rows: int, cols: int, tile_rows: int) -> int:
+    return int(lib().bws_group_tiles(rows, cols, tile_rows))
+
+
+def group_rows(A: Bitmap, tile_rows: int = 2048, max_groups: int = 2, member_row: bool = True,
+               spill_tables: bool = True) -> dict:
+    """generate_groups_large (canonical ascending first fit); dict of tensors."""
+    dev = A.words.device
+    tiles = group_tiles(A.rows, A.cols, tile_rows)
+    G, T = max_groups, tile_rows
+    i32 = dict(dtype=torch.int32, device=dev)
+    out = dict(
+        group_id=torch.empty((A.rows, A.ldw), **i32),
+        group_mask=torch.empty((tiles, G), **i32),
+        group_vals=torch.empty((tiles, G, 32), dtype=torch.float32, device=dev),
+        member_row=torch.empty((tiles, G, 32), **i32) if member_row else None,
+        spill_cnt=torch.empty(tiles, **i32),
+        spill_nnz=torch.empty(tiles, **i32),
+        spill_table=torch.empty((tiles, T), **i32) if spill_tables else None,
+        spill_rev=torch.empty((tiles, T), **i32) if spill_tables else None,
+        grouped_nnz=torch.empty(1, dtype=torch.int64, device=dev),
+    )
+    g = _Groups(*[None if out[k] is None else out[k].data_ptr() for k in (
+        "group_id", "group_mask", "group_vals", "member_row", "spill_cnt", "spill_nnz",
+        "spill_table", "spill_rev", "grouped_nnz")])
+    s = A.struct()
+    _check(lib().bws_group_rows(C.byref(s), tile_rows, max_groups, C.byref(g), _stream(dev)),
+           "bws_group_rows")
+    return out
+
+
+def group_spilled_csr(A: Bitmap, tile_rows: int, groups: dict) -> dict:
+    dev = A.words.device
+    tiles = group_tiles(A.rows, A.cols, tile_rows)
+    nrow = int(groups["spill_cnt"].sum().item())
+    nnz = int(groups["spill_nnz"].sum().item())
+    i32 = dict(dtype=torch.int32, device=dev)
+    out = dict(spill_row_off=torch.empty(tiles + 1, **i32),
+               spill_nnz_off=torch.empty(tiles + 1, **i32),
+               row_end=torch.empty(max(nrow, 1), **i32),
+               col=torch.empty(max(nnz, 1), **i32),
+               val=torch.empty(max(nnz, 1), dtype=torch.float32, device=dev))
+    ws = _workspace(lib().bws_group_spilled_workspace_size(A.rows, A.cols, tile_rows), dev)
+    s = A.struct()
+    _check(lib().bws_group_spilled_csr(
+        C.byref(s), tile_rows, _ptr(groups["spill_cnt"]), _ptr(groups["spill_nnz"]),
+        _ptr(groups["spill_table"]), _ptr(out["spill_row_off"]), _ptr(out["spill_nnz_off"]),
+        _ptr(out["row_end"]), _ptr(out["col"]), _ptr(out["val"]), _ptr(ws), ws.numel(),
+        _stream(dev)), "bws_group_spilled_csr")
+    out["row_end"] = out["row_end"][:nrow]
+    out["col"] = out["col"][:nnz]
+    out["val"] = out["val"][:nnz]
+    return out
+
+
+def group_scatter(A: Bitmap, groups: dict, tile_rows: int, max_groups: int,
+                  D: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """group2dense: write group values into D at every grouped set bit."""
+    dev = A.words.device
+    if D is None:
+        D = torch.zeros((A.rows, A.cols), dtype=torch.float32, device=dev)
+    _check(lib().bws_group_scatter(_ptr(A.words), A.rows, A.cols, A.ldw, _ptr(groups["group_id"]),
+                                   _ptr(groups["group_vals"]), tile_rows, max_groups, _ptr(D),
+                                   D.stride(0), _stream(dev)), "bws_group_scatter")
+    return D
+
+
+def group_gather(D: torch.Tensor, A: Bitmap, groups: dict, tile_rows: int, max_groups: int,
+                 group_vals: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """dense2group_from_idx: read group values back from D."""
+    dev = A.words.device
+    gv = groups["group_vals"].clone() if group_vals is None else group_vals
+    _check(lib().bws_group_gather(_ptr(D), D.stride(0), _ptr(A.words), A.rows, A.cols, A.ldw,
+                                  _ptr(groups["group_id"]), tile_rows, max_groups, _ptr(gv),
+                                  _stream(dev)), "bws_group_gather")
+    return gv
+
+
+# ----------------------------------------------------------------------------
+# (3) grouped tile GEMM
+# ----------------------------------------------------------------------------
+def make_plan(kernel: int = 0, rows_per_wg: int = 0, n_per_wg: int = 0, mfma_min_bits: int = 0):
+    return _Plan(kernel, rows_per_wg, n_per_wg, mfma_min_bits, (C.c_int32 * 4)())
+
+
+def spmm(A: Bitmap, B: torch.Tensor, precision: str = "fp32", alpha: float = 1.0,
+         beta: float = 0.0, out: Optional[torch.Tensor] = None, plan=None) -> torch.Tensor:
+    """C = alpha * A·B + beta * C (C FP32, M x N)."""
+    _require_gpu(B)
+    assert B.dim() == 2 and (B.stride(1) == 1 or B.shape[1] <= 1) and B.shape[0] == A.cols
+    N = B.shape[1]
+    dev = B.device
+    if out is None:
+        out = torch.empty((A.rows, N), dtype=torch.float32, device=dev)
+        if beta != 0.0:
+            raise ValueError("beta != 0 needs `out`")
+    s = A.struct()
+    _check(lib().bws_spmm(C.byref(s), _ptr(B), DTYPE[B.dtype], B.stride(0), N, _ptr(out),
+                          out.stride(0), alpha, beta, PRECISION[precision],
+                          None if plan is None else C.byref(plan), None, 0, _stream(dev)),
+           "bws_spmm")
+    return out
+
+
+def spmm_tA(AT: Bitmap, dC: torch.Tensor, precision: str = "fp32", alpha: float = 1.0,
+            beta: float = 0.0, out: Optional[torch.Tensor] = None, plan=None) -> torch.Tensor:
+    """dB = alpha * Aᵀ·dC + beta * dB, given AT = transpose(A)."""
+    _require_gpu(dC)
+    N = dC.shape[1]
+    dev = dC.device
+    if out is None:
+        out = torch.empty((AT.rows, N), dtype=torch.float32, device=dev)
+    s = AT.struct()
+    _check(lib().bws_spmm_tA(C.byref(s), _ptr(dC), DTYPE[dC.dtype], dC.stride(0), N, _ptr(out),
+                             out.stride(0), alpha, beta, PRECISION[precision],
+                             None if plan is None else C.byref(plan), None, 0, _stream(dev)),
+           "bws_spmm_tA")
+    return out
+
+
+def spmm_groups(A: Bitmap, groups: dict, tile_rows: int, max_groups: int,
+                B: torch.Tensor) -> torch.Tensor:
+    """C = A·B with values read from the group tables (grouped evaluation)."""
+    N = B.shape[1]
+    out = torch.empty((A.rows, N), dtype=torch.float32, device=B.device)
+    s = A.struct()
+    _check(lib().bws_spmm_groups(C.byref(s), _ptr(groups["group_id"]), _ptr(groups["group_vals"]),
+                                 tile_rows, max_groups, _ptr(B), B.stride(0), N, _ptr(out),
+                                 out.stride(0), _stream(B.device)), "bws_spmm_groups")
+    return out
+
+
+def sddmm(A: Bitmap, dC: torch.Tensor, B: torch.Tensor, precision: str = "fp32") -> torch.Tensor:
+    """dA values = (dC·Bᵀ) at A's set bits, in A's value order."""
+    _require_gpu(dC, B)
+    rp = A.row_ptr if A.row_ptr is not None else row_ptr(A)
+    nnz = int(rp[-1].item())
+    out = torch.empty(max(nnz, 1), dtype=torch.float32, device=B.device)
+    bm = Bitmap(A.words, A.rows, A.cols, rp, None)
+    s = bm.struct()
+    _check(lib().bws_sddmm(C.byref(s), _ptr(dC), _ptr(B), DTYPE[B.dtype], dC.stride(0),
+                           B.stride(0), B.shape[1], _ptr(out), PRECISION[precision],
+                           _stream(B.device)), "bws_sddmm")
+    return out[:nnz]
+
+
+def spmm_batched(problems, precision: str = "fp32", plan=None):
+    """problems: list of (A: Bitmap, B: Tensor, C: Tensor); C = A·B for each."""
+    n = len(problems)
+    arr = (_Problem * max(n, 1))()
+    dtype = None
+    dev = None
+    for i, (A, B, Cm) in enumerate(problems):
+        arr[i] = _Problem(A.struct(), B.data_ptr(), B.stride(0), B.shape[1], Cm.data_ptr(),
+                          Cm.stride(0))
+        dtype = DTYPE[B.dtype]
+        dev = B.device
+    if n == 0:
+        return
+    ws = _workspace(lib().bws_spmm_batched_workspace_size(n), dev)
+    _check(lib().bws_spmm_batched(arr, n, dtype, PRECISION[precision],
+                                  None if plan is None else C.byref(plan), _ptr(ws), ws.numel(),
+                                  _stream(dev)), "bws_spmm_batched")
+
+
+# ----------------------------------------------------------------------------
+# host generators (no GPU)
+# ----------------------------------------------------------------------------
+def gen_bitmap_iid(seed: int, rows: int, cols: int, p: float):
+    import numpy as np
+    W = np.empty((rows, words_for(cols)), np.uint32)
+    _check(lib().bws_gen_bitmap_iid(seed, rows, cols, p, W.ctypes.data_as(C.c_void_p), W.shape[1]),
+           "bws_gen_bitmap_iid")
+    return W
+
+
+def gen_bitmap_clustered(seed: int, rows: int, cols: int, n_proto: int, p_proto: float,
+                         flip: float):
+    import numpy as np
+    W = np.empty((rows, words_for(cols)), np.uint32)
+    _check(lib().bws_gen_bitmap_clustered(seed, rows, cols, n_proto, p_proto, flip,
+                                          W.ctypes.data_as(C.c_void_p), W.shape[1]),
+           "bws_gen_bitmap_clustered")
+    return W
+
+
+def gen_f32(seed: int, shape, kind: str = "normal", a: float = 0.0, b: float = 1.0):
+    """kind: 'normal' N(a, b^2), 'uniform' U[a, b), 'logu' random sign * 10^U[a, b)."""
+    import numpy as np
+    out = np.empty(shape, np.float32)
+    k = {"normal": 0, "uniform": 1, "logu": 2}[kind]
+    _check(lib().bws_gen_f32(seed, out.size, k, a, b, out.ctypes.data_as(C.c_void_p)),
+           "bws_gen_f32")
+    return out
+
+
+# reference-named aliases
+dense2bitmask = pack_bitmap
+layout_trans = unpack_bitplanes_f32
+uint8_layout_trans = unpack_bitplanes_u8
+generate_groups_large = group_rows
+generate_spilled_csr = group_spilled_csr
+group2dense = group_scatter
+dense2group_from_idx = group_gather
+spgemm_compute = spmm

@@ -1,0 +1,112 @@
+// bws_internal.hpp — shared host/device helpers of libbws (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bws.h"
+
+namespace bws {
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void set_last_hip_error(int32_t e);
+
+inline bws_status hip_check(hipError_t e) {
+  if (e != hipSuccess) {
+    set_last_hip_error((int32_t)e);
+    return BWS_EHIP;
+  }
+  return BWS_OK;
+}
+
+// Status of the launches just enqueued (launch-configuration errors only).
+inline bws_status launch_check() { return hip_check(hipGetLastError()); }
+
+__host__ __device__ inline int64_t words_for(int64_t cols) { return (cols + 31) / 32; }
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+#define BWS_TRY(expr)                    \
+  do {                                   \
+    bws_status _s = (expr);              \
+    if (_s != BWS_OK) return _s;         \
+  } while (0)
+
+// Validation helpers (host only, no HIP calls).
+inline bool dims_ok(int64_t rows, int64_t cols) { return rows >= 0 && cols >= 0; }
+inline bool ldw_ok(int64_t cols, int64_t ldw) { return ldw >= words_for(cols) && ldw >= 1; }
+inline bool ld_ok(int64_t cols, int64_t ld) { return ld >= cols && ld >= 1; }
+bws_status check_bitmap(const bws_bitmap* A, bool need_row_ptr);
+
+// Maximum grid dimension used for 1-D launches (blocks are looped over).
+constexpr int64_t kMaxGrid = 1 << 30;
+
+// Device scratch carving: a linear allocator over the caller's workspace.
+struct Carve {
+  char* base;
+  size_t used = 0;
+  explicit Carve(void* p) : base(static_cast<char*>(p)) {}
+  template <class T>
+  T* take(size_t n) {
+    used = align_up(used, 256);
+    T* p = reinterpret_cast<T*>(base ? base + used : nullptr);
+    used += n * sizeof(T);
+    return p;
+  }
+};
+
+// Internal launches shared between translation units.
+bws_status launch_word_offsets(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                               int32_t* woff, hipStream_t s);
+bws_status launch_row_ptr(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                          int64_t* row_ptr, void* ws, hipStream_t s);
+bws_status launch_exclusive_scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws,
+                                     hipStream_t s);
+size_t scan_workspace_bytes(int64_t n);
+
+// ---------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint32_t wbit(int i) { return 0x80000000u >> i; }
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Bits of word w that correspond to columns < cols (MSB-first).
+__device__ __forceinline__ uint32_t valid_bits(int64_t w, int64_t cols) {
+  const int64_t rem = cols - 32 * w;
+  return rem >= 32 ? 0xFFFFFFFFu : (rem <= 0 ? 0u : (0xFFFFFFFFu << (32 - rem)));
+}
+
+// Number of set bits of `m` at columns < i (bits 31 .. 32-i).
+__device__ __forceinline__ int rank_before(uint32_t m, int i) {
+  return i == 0 ? 0 : __popc(m & ~(0xFFFFFFFFu >> i));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T u = __shfl_up(v, o, kWave);
+    if (l >= o) v += u;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
+
+}  // namespace bws

@@ -1,0 +1,302 @@
+// group.hip — row grouping by bitmask (first-fit disjoint groups per tile of
+// tile_rows rows x one 32-column word), spilled-row CSR, un-grouping.
+//
+// One wave owns one tile and walks its rows 64 at a time (lane l <-> row
+// j0 + l).  Within a chunk every pending lane computes its first-fit group
+// against the current group masks; the lowest lane that fits is the next row
+// to join in ascending order, and every pending lane below it must spill
+// (group masks only grow).  Each join adds at least one bit to a 32-bit group
+// mask, so a tile costs at most tile_rows/64 + 32*max_groups wave steps, and
+// the result equals the serial ascending-row first fit exactly.
+#include "bws_internal.hpp"
+
+namespace bws {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void k_zero_i64(int64_t* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+template <bool HAS_VALUES>
+__global__ void __launch_bounds__(kBlock)
+k_group_rows(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t wpr,
+             int64_t ldw,
+             const int64_t* __restrict__ row_ptr, const float* __restrict__ values, int T, int G,
+             int64_t tiles, int32_t* __restrict__ group_id, uint32_t* __restrict__ group_mask,
+             float* __restrict__ group_vals, int32_t* __restrict__ member_row,
+             int32_t* __restrict__ spill_cnt, int32_t* __restrict__ spill_nnz,
+             int32_t* __restrict__ spill_table, int32_t* __restrict__ spill_rev,
+             unsigned long long* __restrict__ grouped_nnz) {
+  const int lane = lane_id();
+  for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < tiles;
+       t += (int64_t)gridDim.x * 4) {
+    const int64_t rt = t / wpr, w = t - rt * wpr;
+    const int64_t row0 = rt * T;
+    const int64_t rem = rows - row0;
+    const int nrows = rem < T ? (int)rem : T;
+    uint32_t gm_lane = 0;  // lane g < G holds the mask of group g
+    int32_t cnt = 0;       // spilled rows so far (wave-uniform)
+    int32_t snnz_lane = 0;
+    for (int j0 = 0; j0 < nrows; j0 += 64) {
+      const int j = j0 + lane;
+      const bool valid = j < nrows;
+      const uint32_t m = valid ? (W[(row0 + j) * ldw + w] & valid_bits(w, cols)) : 0u;
+      uint64_t pending = __ballot(m != 0u);
+      int32_t gid = 0, slot = -1;
+      while (pending) {
+        int fit = G;
+        for (int g = 0; g < G; ++g) {
+          const uint32_t gmg = __shfl(gm_lane, g, kWave);
+          if (fit == G && (gmg & m) == 0u) fit = g;
+        }
+        const bool pend = (pending >> lane) & 1ull;
+        const uint64_t joiners = __ballot(pend && fit < G);
+        const int jl = joiners ? __ffsll((unsigned long long)joiners) - 1 : 64;
+        const uint64_t spillers = joiners ? (pending & ((1ull << jl) - 1ull)) : pending;
+        if ((spillers >> lane) & 1ull) {
+          slot = cnt + __popcll(spillers & ((1ull << lane) - 1ull));
+          gid = -1;
+          snnz_lane += __popc(m);
+        }
+        cnt += __popcll(spillers);
+        pending &= ~spillers;
+        if (!joiners) break;
+        const int gj = __shfl(fit, jl, kWave);
+        const uint32_t mj = __shfl(m, jl, kWave);
+        if (lane == jl) gid = gj;
+        if (lane == gj) gm_lane |= mj;
+        int64_t vbase = 0;
+        if (HAS_VALUES) {
+          const int64_t r = row0 + j0 + jl;
+          int64_t part = 0;
+          for (int64_t x = lane; x < w; x += 64) part += __popc(W[r * ldw + x]);  // full words
+          vbase = row_ptr[r] + wave_sum(part);
+        }
+        if (lane < 32 && (mj & wbit(lane))) {
+          const int64_t idx = (t * G + gj) * 32 + lane;
+          if (member_row) member_row[idx] = j0 + jl;
+          group_vals[idx] = HAS_VALUES ? values[vbase + rank_before(mj, lane)] : 1.0f;
+        }
+        pending &= ~(1ull << jl);
+      }
+      if (valid) {
+        group_id[(row0 + j) * ldw + w] = gid;
+        if (spill_rev) spill_rev[t * T + j] = slot;
+        if (spill_table && slot >= 0) spill_table[t * T + slot] = j;
+      }
+    }
+    if (spill_rev)
+      for (int j = nrows + lane; j < T; j += 64) spill_rev[t * T + j] = -1;
+    if (spill_table)
+      for (int s = cnt + lane; s < T; s += 64) spill_table[t * T + s] = 0;
+    for (int g = 0; g < G; ++g) {
+      const uint32_t gmg = __shfl(gm_lane, g, kWave);
+      if (lane < 32 && !(gmg & wbit(lane))) {
+        const int64_t idx = (t * G + g) * 32 + lane;
+        group_vals[idx] = 0.0f;
+        if (member_row) member_row[idx] = -1;
+      }
+    }
+    if (lane < G) group_mask[t * G + lane] = gm_lane;
+    const int32_t snnz = wave_sum(snnz_lane);
+    const int32_t tile_nnz = wave_sum(lane < G ? __popc(gm_lane) : 0);
+    if (lane == 0) {
+      spill_cnt[t] = cnt;
+      spill_nnz[t] = snnz;
+      if (grouped_nnz) atomicAdd(grouped_nnz, (unsigned long long)tile_nnz);
+    }
+  }
+}
+
+template <bool HAS_VALUES>
+__global__ void __launch_bounds__(kBlock)
+k_group_spilled_csr(const uint32_t* __restrict__ W, int64_t cols, int64_t wpr, int64_t ldw,
+                    const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
+                    const int32_t* __restrict__ woff, int T, int64_t tiles,
+                    const int32_t* __restrict__ spill_cnt, const int32_t* __restrict__ spill_table,
+                    const int32_t* __restrict__ row_off, const int32_t* __restrict__ nnz_off,
+                    int32_t* __restrict__ row_end, int32_t* __restrict__ col,
+                    float* __restrict__ val) {
+  const int lane = lane_id();
+  for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < tiles;
+       t += (int64_t)gridDim.x * 4) {
+    const int64_t rt = t / wpr, w = t - rt * wpr;
+    const int cnt = spill_cnt[t];
+    const int32_t roff = row_off[t], noff = nnz_off[t];
+    int32_t carry = 0;
+    for (int s0 = 0; s0 < cnt; s0 += 64) {
+      const int s = s0 + lane;
+      const bool valid = s < cnt;
+      const int64_t r = rt * T + (valid ? spill_table[t * T + s] : 0);
+      uint32_t m = valid ? (W[r * ldw + w] & valid_bits(w, cols)) : 0u;
+      const int32_t p = __popc(m);
+      const int32_t incl = wave_incl_scan(p);
+      int32_t e = noff + carry + incl - p;
+      if (valid) row_end[roff + s] = carry + incl;
+      int64_t vb = HAS_VALUES && valid ? row_ptr[r] + woff[r * wpr + w] : 0;
+      while (m) {
+        const int i = __clz(m);
+        m &= ~wbit(i);
+        col[e] = i;
+        val[e] = HAS_VALUES ? values[vb++] : 1.0f;
+        ++e;
+      }
+      carry += __shfl(incl, 63, kWave);
+    }
+  }
+}
+
+template <bool GATHER>
+__global__ void __launch_bounds__(kBlock)
+k_group_scatter(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t wpr,
+                int64_t ldw,
+                const int32_t* __restrict__ group_id, float* __restrict__ group_vals, int T,
+                int G, float* __restrict__ D, int64_t ld) {
+  const int64_t n = rows * wpr;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int64_t r = e / wpr, w = e - r * wpr;
+    const int32_t g = group_id[r * ldw + w];
+    if (g < 0) continue;
+    const int64_t t = (r / T) * wpr + w;
+    uint32_t m = W[r * ldw + w] & valid_bits(w, cols);
+    while (m) {
+      const int i = __clz(m);
+      m &= ~wbit(i);
+      float* gv = &group_vals[(t * G + g) * 32 + i];
+      float* d = &D[r * ld + 32 * w + i];
+      if (GATHER) *gv = *d;
+      else *d = *gv;
+    }
+  }
+}
+
+inline int grid_for(int64_t items, int64_t per_block) {
+  int64_t g = ceil_div(items, per_block);
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+}  // namespace
+}  // namespace bws
+
+using namespace bws;
+
+extern "C" {
+
+int64_t bws_group_tiles(int64_t rows, int64_t cols, int32_t tile_rows) {
+  if (rows < 0 || cols < 0 || tile_rows < 1) return -1;
+  return ceil_div(rows, tile_rows) * words_for(cols);
+}
+
+bws_status bws_group_rows(const bws_bitmap* A, int32_t tile_rows, int32_t max_groups,
+                          const bws_groups* out, bws_stream stream) {
+  BWS_TRY(check_bitmap(A, false));
+  if (tile_rows < 1 || tile_rows > 65536 || max_groups < 1 || max_groups > 32 || !out)
+    return BWS_EINVAL;
+  const int64_t tiles = bws_group_tiles(A->rows, A->cols, tile_rows);
+  if (tiles > 0 && (!out->group_id || !out->group_mask || !out->group_vals || !out->spill_cnt ||
+                    !out->spill_nnz))
+    return BWS_EINVAL;
+  if (out->grouped_nnz) {
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, stream, out->grouped_nnz);
+  }
+  if (tiles == 0) return launch_check();
+  const int64_t wpr = words_for(A->cols);
+  auto* gn = reinterpret_cast<unsigned long long*>(out->grouped_nnz);
+  if (A->values) {
+    hipLaunchKernelGGL(k_group_rows<true>, dim3(grid_for(tiles, 4)), dim3(kBlock), 0, stream,
+                       A->words, A->rows, A->cols, wpr, A->ldw, A->row_ptr, A->values, (int)tile_rows,
+                       (int)max_groups, tiles, out->group_id, out->group_mask, out->group_vals,
+                       out->member_row, out->spill_cnt, out->spill_nnz, out->spill_table,
+                       out->spill_rev, gn);
+  } else {
+    hipLaunchKernelGGL(k_group_rows<false>, dim3(grid_for(tiles, 4)), dim3(kBlock), 0, stream,
+                       A->words, A->rows, A->cols, wpr, A->ldw, A->row_ptr, A->values, (int)tile_rows,
+                       (int)max_groups, tiles, out->group_id, out->group_mask, out->group_vals,
+                       out->member_row, out->spill_cnt, out->spill_nnz, out->spill_table,
+                       out->spill_rev, gn);
+  }
+  return launch_check();
+}
+
+size_t bws_group_spilled_workspace_size(int64_t rows, int64_t cols, int32_t tile_rows) {
+  if (rows < 0 || cols < 0 || tile_rows < 1) return 0;
+  const int64_t tiles = bws_group_tiles(rows, cols, tile_rows);
+  Carve c(nullptr);
+  c.take<int32_t>((size_t)(rows * words_for(cols)));
+  c.take<char>(scan_workspace_bytes(tiles));
+  return c.used + 256;
+}
+
+bws_status bws_group_spilled_csr(const bws_bitmap* A, int32_t tile_rows,
+                                 const int32_t* spill_cnt, const int32_t* spill_nnz,
+                                 const int32_t* spill_table, int32_t* spill_row_off,
+                                 int32_t* spill_nnz_off, int32_t* row_end, int32_t* col,
+                                 float* val, void* workspace, size_t workspace_bytes,
+                                 bws_stream stream) {
+  BWS_TRY(check_bitmap(A, false));
+  if (tile_rows < 1 || tile_rows > 65536) return BWS_EINVAL;
+  const int64_t tiles = bws_group_tiles(A->rows, A->cols, tile_rows);
+  if (!spill_row_off || !spill_nnz_off) return BWS_EINVAL;
+  if (tiles > 0 && (!spill_cnt || !spill_nnz || !spill_table || !row_end || !col || !val))
+    return BWS_EINVAL;
+  if (!workspace ||
+      workspace_bytes < bws_group_spilled_workspace_size(A->rows, A->cols, tile_rows))
+    return BWS_EWORKSPACE;
+  const int64_t wpr = words_for(A->cols);
+  Carve c(workspace);
+  int32_t* woff = c.take<int32_t>((size_t)(A->rows * wpr));
+  void* scan_ws = c.take<char>(scan_workspace_bytes(tiles));
+  BWS_TRY(launch_exclusive_scan_i32(spill_cnt, tiles, spill_row_off, scan_ws, stream));
+  BWS_TRY(launch_exclusive_scan_i32(spill_nnz, tiles, spill_nnz_off, scan_ws, stream));
+  if (tiles == 0) return BWS_OK;
+  if (A->values) {
+    BWS_TRY(launch_word_offsets(A->words, A->rows, A->cols, A->ldw, woff, stream));
+    hipLaunchKernelGGL(k_group_spilled_csr<true>, dim3(grid_for(tiles, 4)), dim3(kBlock), 0,
+                       stream, A->words, A->cols, wpr, A->ldw, A->row_ptr, A->values, woff,
+                       (int)tile_rows, tiles, spill_cnt, spill_table, spill_row_off,
+                       spill_nnz_off, row_end, col, val);
+  } else {
+    hipLaunchKernelGGL(k_group_spilled_csr<false>, dim3(grid_for(tiles, 4)), dim3(kBlock), 0,
+                       stream, A->words, A->cols, wpr, A->ldw, A->row_ptr, A->values, woff,
+                       (int)tile_rows, tiles, spill_cnt, spill_table, spill_row_off,
+                       spill_nnz_off, row_end, col, val);
+  }
+  return launch_check();
+}
+
+bws_status bws_group_scatter(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                             const int32_t* group_id, const float* group_vals,
+                             int32_t tile_rows, int32_t max_groups, float* D, int64_t ld,
+                             bws_stream stream) {
+  if (!dims_ok(rows, cols) || !ldw_ok(cols, ldw) || !ld_ok(cols, ld)) return BWS_EINVAL;
+  if (tile_rows < 1 || tile_rows > 65536 || max_groups < 1 || max_groups > 32) return BWS_EINVAL;
+  if (rows == 0 || cols == 0) return BWS_OK;
+  if (!W || !group_id || !group_vals || !D) return BWS_EINVAL;
+  const int64_t wpr = words_for(cols);
+  hipLaunchKernelGGL(k_group_scatter<false>, dim3(grid_for(rows * wpr, kBlock)), dim3(kBlock), 0,
+                     stream, W, rows, cols, wpr, ldw, group_id, const_cast<float*>(group_vals),
+                     (int)tile_rows, (int)max_groups, D, ld);
+  return launch_check();
+}
+
+bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64_t rows,
+                            int64_t cols, int64_t ldw, const int32_t* group_id,
+                            int32_t tile_rows, int32_t max_groups, float* group_vals,
+                            bws_stream stream) {
+  if (!dims_ok(rows, cols) || !ldw_ok(cols, ldw) || !ld_ok(cols, ld)) return BWS_EINVAL;
+  if (tile_rows < 1 || tile_rows > 65536 || max_groups < 1 || max_groups > 32) return BWS_EINVAL;
+  if (rows == 0 || cols == 0) return BWS_OK;
+  if (!W || !group_id || !group_vals || !D) return BWS_EINVAL;
+  const int64_t wpr = words_for(cols);
+  hipLaunchKernelGGL(k_group_scatter<true>, dim3(grid_for(rows * wpr, kBlock)), dim3(kBlock), 0,
+                     stream, W, rows, cols, wpr, ldw, group_id, group_vals, (int)tile_rows,
+                     (int)max_groups, const_cast<float*>(D), ld);
+  return launch_check();
+}
+
+}  // extern "C"

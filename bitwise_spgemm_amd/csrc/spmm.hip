@@ -1,0 +1,177 @@
+// spmm.hip — C-ABI dispatch of the grouped tile GEMM family: argument
+// validation on the host, kernel selection, and the small helper kernels.
+#include "bws_internal.hpp"
+
+namespace bws {
+
+bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
+                     float* C, int64_t ldc, float alpha, float beta, int32_t rows_per_wave,
+                     hipStream_t s);
+bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
+                     float* C, int64_t ldc, float alpha, float beta, int32_t precision,
+                     hipStream_t s);
+bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
+                              int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
+                              float* C, int64_t ldc, hipStream_t s);
+bws_status sddmm_f32_launch(const bws_bitmap* A, const float* dC, int64_t lddc, const float* B,
+                            int64_t ldb, int64_t N, float* dA, hipStream_t s);
+bws_status sddmm_mfma_launch(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
+                             int64_t lddc, int64_t ldb, int64_t N, float* dA, int32_t precision,
+                             void* ws, hipStream_t s);
+size_t sddmm_mfma_workspace(const bws_bitmap* A);
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+k_scale(float* __restrict__ C, int64_t M, int64_t N, int64_t ldc, float beta) {
+  const int64_t n = M * N;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t r = e / N, c = e - r * N;
+    float* p = C + r * ldc + c;
+    *p = beta != 0.0f ? beta * *p : 0.0f;
+  }
+}
+
+bool dtype_ok(int32_t d) { return d == BWS_DTYPE_F32 || d == BWS_DTYPE_BF16 || d == BWS_DTYPE_F16; }
+bool prec_ok(int32_t p) { return p >= BWS_PREC_FP32 && p <= BWS_PREC_BF16X2; }
+
+// precision/b_dtype combinations accepted by bws_spmm
+bool combo_ok(int32_t precision, int32_t b_dtype) {
+  switch (precision) {
+    case BWS_PREC_FP32: return true;  // 16-bit inputs widen exactly
+    case BWS_PREC_BF16: return b_dtype == BWS_DTYPE_BF16 || b_dtype == BWS_DTYPE_F32;
+    case BWS_PREC_FP16: return b_dtype == BWS_DTYPE_F16 || b_dtype == BWS_DTYPE_F32;
+    case BWS_PREC_BF16X2: return b_dtype == BWS_DTYPE_F32;
+    default: return false;
+  }
+}
+
+size_t esize(int32_t d) { return d == BWS_DTYPE_F32 ? 4 : 2; }
+
+}  // namespace
+
+bws_status spmm_dispatch(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb,
+                         int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                         int32_t precision, const bws_plan* plan, hipStream_t s) {
+  const int64_t M = A->rows, K = A->cols;
+  if (M == 0 || N == 0) return BWS_OK;
+  if (K == 0) {
+    int64_t g = ceil_div(M * N, 256);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)g), dim3(256), 0, s, C, M, N, ldc, beta);
+    return launch_check();
+  }
+  int32_t kernel = plan ? plan->kernel : 0;
+  if (precision == BWS_PREC_FP32) {
+    if (kernel == 2) return BWS_EUNSUPPORTED;
+    kernel = 1;
+  } else if (kernel == 0) {
+    // MFMA for 0/1 masks; value-carrying masks take the FP32 vector path,
+    // which is at least as accurate as the requested 16-bit precision.
+    kernel = A->values ? 1 : 2;
+  }
+  if (kernel == 2) {
+    if (A->values) return BWS_EUNSUPPORTED;
+    return spmm_mfma(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, s);
+  }
+  int32_t rpw = plan && plan->rows_per_wg ? plan->rows_per_wg / 8 : 0;
+  if (rpw == 0) {
+    const int64_t tiles32 = ceil_div(M, 256) * ceil_div(N, 256);
+    rpw = tiles32 >= 192 ? 32 : 8;
+  }
+  return spmm_valu(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, rpw, s);
+}
+
+}  // namespace bws
+
+using namespace bws;
+
+extern "C" {
+
+size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision,
+                               const bws_plan* plan) {
+  (void)A;
+  (void)N;
+  (void)precision;
+  (void)plan;
+  return 0;
+}
+
+bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
+                    float* C, int64_t ldc, float alpha, float beta, int32_t precision,
+                    const bws_plan* plan, void* workspace, size_t workspace_bytes,
+                    bws_stream stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  BWS_TRY(check_bitmap(A, false));
+  if (N < 0 || !dtype_ok(b_dtype) || !prec_ok(precision)) return BWS_EINVAL;
+  if (!combo_ok(precision, b_dtype)) return BWS_EUNSUPPORTED;
+  if (plan && (plan->kernel < 0 || plan->kernel > 2 || plan->rows_per_wg < 0 ||
+               (plan->rows_per_wg != 0 && plan->rows_per_wg != 64 && plan->rows_per_wg != 256)))
+    return BWS_EINVAL;
+  if (A->rows == 0 || N == 0) return BWS_OK;
+  if (!C || ldc < N) return BWS_EINVAL;
+  if (A->cols > 0 && (!B || ldb < N)) return BWS_EINVAL;
+  if ((reinterpret_cast<uintptr_t>(C) & 3u) ||
+      (reinterpret_cast<uintptr_t>(B) & (esize(b_dtype) - 1)))
+    return BWS_EINVAL;
+  return spmm_dispatch(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, plan, stream);
+}
+
+bws_status bws_spmm_tA(const bws_bitmap* AT, const void* dC, int32_t dc_dtype, int64_t lddc,
+                       int64_t N, float* dB, int64_t lddb, float alpha, float beta,
+                       int32_t precision, const bws_plan* plan, void* workspace,
+                       size_t workspace_bytes, bws_stream stream) {
+  return bws_spmm(AT, dC, dc_dtype, lddc, N, dB, lddb, alpha, beta, precision, plan, workspace,
+                  workspace_bytes, stream);
+}
+
+bws_status bws_spmm_groups(const bws_bitmap* A, const int32_t* group_id, const float* group_vals,
+                           int32_t tile_rows, int32_t max_groups, const float* B, int64_t ldb,
+                           int64_t N, float* C, int64_t ldc, bws_stream stream) {
+  BWS_TRY(check_bitmap(A, false));
+  if (N < 0 || tile_rows < 1 || tile_rows > 65536 || max_groups < 1 || max_groups > 32)
+    return BWS_EINVAL;
+  if (A->rows == 0 || N == 0) return BWS_OK;
+  if (!C || ldc < N) return BWS_EINVAL;
+  if (A->cols > 0 && (!B || ldb < N || !group_id || !group_vals)) return BWS_EINVAL;
+  return spmm_groups_launch(A, group_id, group_vals, tile_rows, max_groups, B, ldb, N, C, ldc,
+                            stream);
+}
+
+bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
+                     int64_t lddc, int64_t ldb, int64_t N, float* dA_values, int32_t precision,
+                     bws_stream stream) {
+  BWS_TRY(check_bitmap(A, true));
+  if (N < 0 || !dtype_ok(dtype) || !prec_ok(precision)) return BWS_EINVAL;
+  if (A->rows == 0 || A->cols == 0) return BWS_OK;
+  if (!dA_values) return BWS_EINVAL;
+  if (N > 0 && (!dC || !B || lddc < N || ldb < N)) return BWS_EINVAL;
+  if (precision == BWS_PREC_FP32) {
+    if (dtype != BWS_DTYPE_F32) return BWS_EUNSUPPORTED;
+    return sddmm_f32_launch(A, static_cast<const float*>(dC), lddc,
+                            static_cast<const float*>(B), ldb, N, dA_values, stream);
+  }
+  return BWS_EUNSUPPORTED;
+}
+
+size_t bws_spmm_batched_workspace_size(int32_t count) {
+  return count < 0 ? 0 : align_up((size_t)count * sizeof(bws_problem), 256) + 256;
+}
+
+bws_status bws_spmm_batched(const bws_problem* problems, int32_t count, int32_t b_dtype,
+                            int32_t precision, const bws_plan* plan, void* workspace,
+                            size_t workspace_bytes, bws_stream stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  if (count < 0 || (count > 0 && !problems)) return BWS_EINVAL;
+  for (int32_t i = 0; i < count; ++i) {
+    const bws_problem& p = problems[i];
+    BWS_TRY(bws_spmm(&p.A, p.B, b_dtype, p.ldb, p.N, p.C, p.ldc, 1.0f, 0.0f, precision, plan,
+                     nullptr, 0, stream));
+  }
+  return BWS_OK;
+}
+
+}  // extern "C"

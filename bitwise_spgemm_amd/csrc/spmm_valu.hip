@@ -1,0 +1,379 @@
+// spmm_valu.hip — FP32 vector-add tile GEMM: C = alpha * A·B + beta * C with a
+// bitmap A (0/1 or value-carrying) and a dense B, for gfx950.
+//
+// Tile: a workgroup of 8 waves owns 8*RPW rows x 256 columns of C.  K is
+// walked one 32-column word at a time: the matching 32 x 256 panel of B is
+// staged in LDS (double-buffered, coalesced 16-byte global loads), and every
+// wave walks the set bits of its rows' words with wave-uniform scalar bit
+// scans (s_flbit / s_andn2), so there is no divergence: each set bit (r, k)
+// adds LDS row k (4 floats per lane) into row r's accumulators ("replace the
+// multiply with an add", backup/sgemm.cu:428-436, backup/layout_trans.cu:158-171).
+// Workgroups are mapped XCD-aware: consecutive tiles of one 256-column panel
+// go to the same XCD, so that panel of B is fetched from HBM about once per
+// XCD and re-read from that XCD's L2.
+#include "bws_internal.hpp"
+
+namespace bws {
+namespace {
+
+constexpr int kWaves = 8;
+constexpr int kThreads = kWaves * 64;
+constexpr int kBN = 256;  // columns per workgroup (4 per lane)
+constexpr int kKC = 32;   // K rows per stage (one bitmap word)
+
+// B element types: 0 = f32, 1 = bf16 (stored as uint16), 2 = f16.
+template <int TB>
+struct BType;
+template <>
+struct BType<0> {
+  using T = float;
+  static __device__ __forceinline__ float cvt(T x) { return x; }
+};
+template <>
+struct BType<1> {
+  using T = uint16_t;
+  static __device__ __forceinline__ float cvt(T x) { return __uint_as_float((uint32_t)x << 16); }
+};
+template <>
+struct BType<2> {
+  using T = _Float16;
+  static __device__ __forceinline__ float cvt(T x) { return (float)x; }
+};
+
+template <int TB, bool VEC>
+__device__ __forceinline__ float4 load_b4(const typename BType<TB>::T* __restrict__ B, int64_t k,
+                                          int64_t n, int64_t ldb, int64_t K, int64_t N) {
+  using T = typename BType<TB>::T;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (k >= K) return v;
+  const T* p = B + k * ldb + n;
+  if (VEC && n + 3 < N) {
+    if (TB == 0) {
+      v = *reinterpret_cast<const float4*>(p);
+    } else {
+      const uint2 u = *reinterpret_cast<const uint2*>(p);
+      T e[4];
+      __builtin_memcpy(e, &u, 8);
+      v = make_float4(BType<TB>::cvt(e[0]), BType<TB>::cvt(e[1]), BType<TB>::cvt(e[2]),
+                      BType<TB>::cvt(e[3]));
+    }
+  } else {
+    if (n + 0 < N) v.x = BType<TB>::cvt(p[0]);
+    if (n + 1 < N) v.y = BType<TB>::cvt(p[1]);
+    if (n + 2 < N) v.z = BType<TB>::cvt(p[2]);
+    if (n + 3 < N) v.w = BType<TB>::cvt(p[3]);
+  }
+  return v;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ void fma4(float4& acc, float a, const float4& b) {
+  acc.x = fmaf(a, b.x, acc.x);
+  acc.y = fmaf(a, b.y, acc.y);
+  acc.z = fmaf(a, b.z, acc.z);
+  acc.w = fmaf(a, b.w, acc.w);
+}
+__device__ __forceinline__ void add4(float4& acc, const float4& b) {
+  acc.x += b.x;
+  acc.y += b.y;
+  acc.z += b.z;
+  acc.w += b.w;
+}
+
+// Tile index -> (row block, column block), XCD-aware: workgroup g runs on
+// XCD g % 8; XCD x receives the contiguous tile range [x*per_xcd, (x+1)*per_xcd)
+// of the column-block-major tile order.
+__device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t per_xcd) {
+  return (g & 7) * per_xcd + (g >> 3);
+}
+
+template <int TB, int RPW, bool VALS, bool VEC>
+__global__ void __launch_bounds__(kThreads)
+k_spmm_rows(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+            const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
+            const typename BType<TB>::T* __restrict__ B, int64_t ldb, int64_t N,
+            float* __restrict__ C, int64_t ldc, float alpha, float beta, int64_t row_blocks,
+            int64_t col_blocks, int64_t per_xcd) {
+  __shared__ float4 sB[2][kKC][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
+  const int64_t n0 = cb * kBN;
+  const int64_t mw = rb * (kWaves * RPW) + (int64_t)wave * RPW;
+  const int64_t nk = (K + kKC - 1) / kKC;
+  const int krem = (int)(K % kKC);
+  const uint32_t last_mask = krem ? (0xFFFFFFFFu << (32 - krem)) : 0xFFFFFFFFu;
+  const bool row_lane = lane < RPW && mw + lane < M;
+
+  float4 acc[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  float4 st[kKC * 64 / kThreads];
+  auto load_chunk = [&](int64_t kc) {
+#pragma unroll
+    for (int s = 0; s < kKC * 64 / kThreads; ++s) {
+      const int idx = threadIdx.x + kThreads * s;
+      st[s] = load_b4<TB, VEC>(B, kc * kKC + (idx >> 6), n0 + 4 * (idx & 63), ldb, K, N);
+    }
+  };
+  auto store_chunk = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < kKC * 64 / kThreads; ++s) {
+      const int idx = threadIdx.x + kThreads * s;
+      sB[buf][idx >> 6][idx & 63] = st[s];
+    }
+  };
+
+  load_chunk(0);
+  store_chunk(0);
+  uint32_t wcur = row_lane ? W[(mw + lane) * ldw] : 0u;
+  int64_t e_l = 0;
+  if (VALS) e_l = row_lane ? row_ptr[mw + lane] : 0;
+  __syncthreads();
+
+  for (int64_t kc = 0; kc < nk; ++kc) {
+    const int buf = (int)(kc & 1);
+    const bool more = kc + 1 < nk;
+    if (more) load_chunk(kc + 1);
+    const uint32_t wnext = (more && row_lane) ? W[(mw + lane) * ldw + kc + 1] : 0u;
+    const uint32_t wc = more ? wcur : (wcur & last_mask);
+
+    float vv[VALS ? RPW : 1];
+    if (VALS) {
+      const int cnt_l = __popc(wc);
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        const int64_t e = readlane_i64(e_l, r);
+        const int c = __builtin_amdgcn_readlane(cnt_l, r);
+        vv[r] = lane < c ? values[e + lane] : 0.0f;
+      }
+      e_l += cnt_l;
+    }
+
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)wc, r);
+      int j = 0;
+      while (m) {
+        const int i = __clz((int)m);
+        m &= ~wbit(i);
+        const float4 b = sB[buf][i][lane];
+        if (VALS) {
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b);
+          ++j;
+        } else {
+          add4(acc[r], b);
+        }
+      }
+    }
+    if (more) store_chunk(buf ^ 1);
+    wcur = wnext;
+    __syncthreads();
+  }
+
+  const int64_t n = n0 + 4 * lane;
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int64_t row = mw + r;
+    if (row >= M) break;
+    float* c = C + row * ldc + n;
+    float4 v = acc[r];
+    v.x *= alpha;
+    v.y *= alpha;
+    v.z *= alpha;
+    v.w *= alpha;
+    if (VEC && n + 3 < N) {
+      if (beta != 0.0f) {
+        const float4 o = *reinterpret_cast<const float4*>(c);
+        v.x = fmaf(beta, o.x, v.x);
+        v.y = fmaf(beta, o.y, v.y);
+        v.z = fmaf(beta, o.z, v.z);
+        v.w = fmaf(beta, o.w, v.w);
+      }
+      *reinterpret_cast<float4*>(c) = v;
+    } else {
+      const float vs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (n + e < N) c[e] = beta != 0.0f ? fmaf(beta, c[e], vs[e]) : vs[e];
+    }
+  }
+}
+
+// Grouped evaluation: one wave per (row, 256-column block), values from the
+// group tables (group_vals of the row's group in this word) or, for a spilled
+// row, from A's own values.
+__global__ void __launch_bounds__(256)
+k_spmm_groups(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+              const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
+              const int32_t* __restrict__ group_id, const float* __restrict__ gv, int T, int G,
+              const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
+              int64_t ldc, int64_t col_blocks) {
+  const int lane = lane_id();
+  const int64_t wpr = words_for(K);
+  const int64_t total = M * col_blocks;
+  for (int64_t gw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); gw < total;
+       gw += (int64_t)gridDim.x * 4) {
+    const int64_t r = gw / col_blocks, cb = gw - r * col_blocks;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    int64_t e = values ? row_ptr[r] : 0;
+    for (int64_t w = 0; w < wpr; ++w) {
+      uint32_t m = W[r * ldw + w] & valid_bits(w, K);
+      if (!m) continue;
+      const int32_t g = group_id[r * ldw + w];
+      const int64_t t = (r / T) * wpr + w;
+      while (m) {
+        const int i = __clz((int)m);
+        m &= ~wbit(i);
+        const float a = g >= 0 ? gv[(t * G + g) * 32 + i] : (values ? values[e] : 1.0f);
+        ++e;
+        const float* b = B + (32 * w + i) * ldb;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t n = cb * 256 + q * 64 + lane;
+          if (n < N) acc[q] = fmaf(a, b[n], acc[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t n = cb * 256 + q * 64 + lane;
+      if (n < N) C[r * ldc + n] = acc[q];
+    }
+  }
+}
+
+// FP32 SDDMM: dA[row_ptr[r] + j] = Σ_n dC[r, n] B[k_j, n].  One wave per row;
+// the row of dC is held in registers in 1024-column slices.
+template <bool VEC>
+__global__ void __launch_bounds__(256)
+k_sddmm_f32(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+            const int64_t* __restrict__ row_ptr, const float* __restrict__ dC, int64_t lddc,
+            const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ dA) {
+  const int lane = lane_id();
+  const int64_t wpr = words_for(K);
+  for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < M;
+       r += (int64_t)gridDim.x * 4) {
+    const int64_t e0 = row_ptr[r];
+    for (int64_t n0 = 0; n0 < N || n0 == 0; n0 += 1024) {
+      float4 x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        x[q] = load_b4<0, VEC>(dC + r * lddc, 0, n0 + 256 * q + 4 * lane, 0, 1, N);
+      int64_t e = e0;
+      for (int64_t w = 0; w < wpr; ++w) {
+        uint32_t m = W[r * ldw + w] & valid_bits(w, K);
+        while (m) {
+          const int i = __clz((int)m);
+          m &= ~wbit(i);
+          const float* b = B + (32 * w + i) * ldb;
+          float s = 0.f;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float4 y = load_b4<0, VEC>(b, 0, n0 + 256 * q + 4 * lane, 0, 1, N);
+            s = fmaf(x[q].x, y.x, s);
+            s = fmaf(x[q].y, y.y, s);
+            s = fmaf(x[q].z, y.z, s);
+            s = fmaf(x[q].w, y.w, s);
+          }
+          s = wave_sum(s);
+          if (lane == 0) dA[e] = n0 == 0 ? s : dA[e] + s;
+          ++e;
+        }
+      }
+    }
+  }
+}
+
+template <int TB, int RPW, bool VALS, bool VEC>
+bws_status launch_rows(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, float* C,
+                       int64_t ldc, float alpha, float beta, hipStream_t s) {
+  const int64_t row_blocks = ceil_div(A->rows, kWaves * RPW);
+  const int64_t col_blocks = ceil_div(N, kBN);
+  const int64_t tiles = row_blocks * col_blocks;
+  const int64_t per_xcd = ceil_div(tiles, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  hipLaunchKernelGGL((k_spmm_rows<TB, RPW, VALS, VEC>), dim3((unsigned)(8 * per_xcd)),
+                     dim3(kThreads), 0, s, A->words, A->rows, A->cols, A->ldw, A->row_ptr,
+                     A->values, static_cast<const typename BType<TB>::T*>(B), ldb, N, C, ldc,
+                     alpha, beta, row_blocks, col_blocks, per_xcd);
+  return launch_check();
+}
+
+template <int TB, int RPW>
+bws_status launch_rows_v(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, float* C,
+                         int64_t ldc, float alpha, float beta, bool vec, hipStream_t s) {
+  const bool vals = A->values != nullptr;
+  if (vals) {
+    return vec ? launch_rows<TB, RPW, true, true>(A, B, ldb, N, C, ldc, alpha, beta, s)
+               : launch_rows<TB, RPW, true, false>(A, B, ldb, N, C, ldc, alpha, beta, s);
+  }
+  return vec ? launch_rows<TB, RPW, false, true>(A, B, ldb, N, C, ldc, alpha, beta, s)
+             : launch_rows<TB, RPW, false, false>(A, B, ldb, N, C, ldc, alpha, beta, s);
+}
+
+inline int grid_for(int64_t items, int64_t per_block) {
+  int64_t g = ceil_div(items, per_block);
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+}  // namespace
+
+// Vector-add path launcher shared with the dispatcher in spmm.cpp.
+bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
+                     float* C, int64_t ldc, float alpha, float beta, int32_t rows_per_wave,
+                     hipStream_t s) {
+  const size_t esz = b_dtype == BWS_DTYPE_F32 ? 4 : 2;
+  const bool vec = (ldb % 4 == 0) && (ldc % 4 == 0) &&
+                   (reinterpret_cast<uintptr_t>(B) % (4 * esz) == 0) &&
+                   (reinterpret_cast<uintptr_t>(C) % 16 == 0);
+  const bool big = rows_per_wave >= 32;
+#define BWS_ROWS(TB)                                                                   \
+  return big ? launch_rows_v<TB, 32>(A, B, ldb, N, C, ldc, alpha, beta, vec, s)        \
+             : launch_rows_v<TB, 8>(A, B, ldb, N, C, ldc, alpha, beta, vec, s)
+  switch (b_dtype) {
+    case BWS_DTYPE_F32: BWS_ROWS(0);
+    case BWS_DTYPE_BF16: BWS_ROWS(1);
+    case BWS_DTYPE_F16: BWS_ROWS(2);
+    default: return BWS_EUNSUPPORTED;
+  }
+#undef BWS_ROWS
+}
+
+bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
+                              int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
+                              float* C, int64_t ldc, hipStream_t s) {
+  const int64_t col_blocks = ceil_div(N, 256);
+  hipLaunchKernelGGL(k_spmm_groups, dim3(grid_for(A->rows * col_blocks, 4)), dim3(256), 0, s,
+                     A->words, A->rows, A->cols, A->ldw, A->row_ptr, A->values, group_id, gv,
+                     (int)T, (int)G, B, ldb, N, C, ldc, col_blocks);
+  return launch_check();
+}
+
+bws_status sddmm_f32_launch(const bws_bitmap* A, const float* dC, int64_t lddc, const float* B,
+                            int64_t ldb, int64_t N, float* dA, hipStream_t s) {
+  const bool vec = (ldb % 4 == 0) && (lddc % 4 == 0) &&
+                   (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
+                   (reinterpret_cast<uintptr_t>(dC) % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL(k_sddmm_f32<true>, dim3(grid_for(A->rows, 4)), dim3(256), 0, s, A->words,
+                       A->rows, A->cols, A->ldw, A->row_ptr, dC, lddc, B, ldb, N, dA);
+  else
+    hipLaunchKernelGGL(k_sddmm_f32<false>, dim3(grid_for(A->rows, 4)), dim3(256), 0, s, A->words,
+                       A->rows, A->cols, A->ldw, A->row_ptr, dC, lddc, B, ldb, N, dA);
+  return launch_check();
+}
+
+}  // namespace bws

@@ -1,0 +1,328 @@
+/*
+ * bws.h — C ABI of the MI355X (gfx950) bit-wise sparse GEMM library `libbws.so`.
+ *
+ * C = A·B with the sparse operand A held as packed bitmasks and B dense.
+ * The three entry-point families of the reference are kept as a drop-in:
+ *   (1) bitmap layout transform     bws_pack_bitmap_f32, bws_unpack_bitplanes_*, ...
+ *   (2) row grouping by bitmask     bws_group_rows, bws_group_spilled_csr, bws_group_scatter/gather
+ *   (3) grouped tile GEMM           bws_spmm, bws_spmm_groups, bws_spmm_batched (+ adjoints)
+ *
+ * Conventions (ABI-visible, fixed):
+ *   - A bitmap is uint32 words, row-major [rows x ldw], ldw >= ceil(cols/32).
+ *     Bit (31 - i) of word w of row r is set iff element (r, 32*w + i) is nonzero
+ *     (MSB-first, as dense2bitmask writes it, source/src/main.cu:619).  Bits of
+ *     columns >= cols are zero.
+ *   - A value-carrying bitmap adds row_ptr[rows+1] (exclusive prefix of the
+ *     per-row set-bit counts) and values[nnz] stored in (row, column) order,
+ *     i.e. the value of the j-th set bit of row r is values[row_ptr[r] + j]
+ *     (the bitmap + popcount-prefix value layout of source/src/mm.h:83-104 and
+ *     source/src/mm.cu:272, at row granularity).  values == NULL means every
+ *     set bit has value 1.0 (a 0/1 mask).
+ *   - Dense matrices are row-major with a leading dimension in elements.
+ *   - All device pointers are caller-owned; the library never allocates device
+ *     memory on any entry point (workspace is passed in, sized by the matching
+ *     *_workspace_size query).  All work is enqueued on the caller's stream with
+ *     no host synchronisation, so every entry point can be captured into a
+ *     hipGraph.
+ *   - Every entry point returns a bws_status (0 = BWS_OK, negative = error); no
+ *     entry point aborts or prints.  Argument validation happens on the host
+ *     before any HIP call, so invalid calls are rejected without a GPU.
+ */
+#ifndef BWS_H
+#define BWS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t; NULL selects the null stream. */
+typedef struct ihipStream_t* bws_stream;
+
+typedef int32_t bws_status;
+enum {
+  BWS_OK = 0,
+  BWS_EINVAL = -1,       /* bad shape, stride, pointer or enum value            */
+  BWS_ECAPACITY = -2,    /* a size exceeds a documented capacity                */
+  BWS_EUNSUPPORTED = -3, /* valid but unsupported combination (dtype/precision) */
+  BWS_EHIP = -4,         /* a HIP call failed; see bws_last_hip_error()         */
+  BWS_EWORKSPACE = -5,   /* workspace pointer NULL or smaller than required     */
+  BWS_ENODEVICE = -6     /* no usable gfx950 device                             */
+};
+
+typedef enum { BWS_DTYPE_F32 = 0, BWS_DTYPE_BF16 = 1, BWS_DTYPE_F16 = 2 } bws_dtype;
+
+/* Arithmetic used by the product.  FP32 is the drop-in path (FP32 vector adds /
+ * FMAs, FP32 accumulation).  BF16 and FP16 round B (and A's values) to 16 bits
+ * and use MFMA with FP32 accumulation.  BF16X2 splits FP32 B into hi + lo
+ * bfloat16 parts (hi = bf16(b), lo = bf16(b - hi)) and accumulates both
+ * products in FP32 (compensated path). */
+typedef enum {
+  BWS_PREC_FP32 = 0,
+  BWS_PREC_BF16 = 1,
+  BWS_PREC_FP16 = 2,
+  BWS_PREC_BF16X2 = 3
+} bws_precision;
+
+typedef struct {
+  int64_t rows;            /* M                                            */
+  int64_t cols;            /* K                                            */
+  int64_t ldw;             /* words per row stride, >= ceil(cols / 32)     */
+  const uint32_t* words;   /* device [rows x ldw]                          */
+  const int64_t* row_ptr;  /* device [rows + 1] or NULL (required if values) */
+  const float* values;     /* device [nnz] or NULL (0/1 mask)              */
+} bws_bitmap;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Human-readable name of a status code (static storage). */
+const char* bws_status_string(bws_status status);
+
+/* The last hipError_t recorded by this thread when an entry point returned
+ * BWS_EHIP (0 if none). */
+int32_t bws_last_hip_error(void);
+
+/* ABI version: (major << 16) | minor. */
+int32_t bws_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* (1) Bitmap layout transforms                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Pack a dense FP32 matrix into the MSB-first bitmap: bit (31-i) of W[r*ldw+w]
+ * is set iff D[r*ld + 32w+i] != 0.0f (IEEE compare: -0.0 packs to 0, NaN to 1).
+ * Every word of every row (including padding words up to ldw) is written, so W
+ * need not be zeroed first.
+ * Replaces dense2bitmask (source/src/main.cu:591-628). */
+bws_status bws_pack_bitmap_f32(const float* D, int64_t rows, int64_t cols, int64_t ld,
+                               uint32_t* W, int64_t ldw, bws_stream stream);
+
+/* Inverse of the pack for 0/1 masks: D[r, c] = 1.0f if the bit is set, else 0.
+ * Used as the dense operand of round trips and of the dense reference. */
+bws_status bws_unpack_bitmap_f32(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                                 float* D, int64_t ld, bws_stream stream);
+
+/* Unpacked bit-planes of the raw FP32 bit pattern: dst[b*n + i] = (bits(src[i]) >> b) & 1
+ * for b < bit_width (1..32).  Replaces layout_trans (backup/layout_trans.cu:75-85). */
+bws_status bws_unpack_bitplanes_f32(const float* src, int64_t n, int32_t bit_width,
+                                    int32_t* dst, bws_stream stream);
+
+/* Unpacked bit-planes of uint8 values: dst[b*n + i] = (src[i] >> b) & 1, b < 8.
+ * Replaces uint8_layout_trans (backup/layout_trans.cu:87-97). */
+bws_status bws_unpack_bitplanes_u8(const uint8_t* src, int64_t n, int32_t* dst,
+                                   bws_stream stream);
+
+/* Packed bit-planes of an int8 matrix: plane b (0..7) of the two's-complement
+ * pattern of D is packed exactly like bws_pack_bitmap_f32 into
+ * P[b*rows*ldw + r*ldw + w].  Σ_b w_b·plane_b reproduces D with the weights
+ * [1,2,4,8,16,32,64,-128] of bws_bitplane_weights. */
+bws_status bws_pack_bitplanes_i8(const int8_t* D, int64_t rows, int64_t cols, int64_t ld,
+                                 uint32_t* P, int64_t ldw, bws_stream stream);
+
+/* Two's-complement plane weights for int8 (host, no GPU): w[b] = 2^b for b<7,
+ * w[7] = -128.  The reference's bit-weight multiplicand row
+ * (initialize_multiplicand, source/src/main.cu:889-912). */
+bws_status bws_bitplane_weights(float* w8);
+
+/* Per-plane per-row set-bit counts of packed planes: counts[b*rows + r] =
+ * popcount of row r of plane b (b < planes).  The intent of layout_compact
+ * (backup/layout_trans.cu:100-118), computed on packed planes. */
+bws_status bws_bitplane_row_counts(const uint32_t* P, int32_t planes, int64_t rows,
+                                   int64_t cols, int64_t ldw, int32_t* counts,
+                                   bws_stream stream);
+
+/* row_ptr[r] = number of set bits in rows < r (row_ptr[rows] = nnz), with an
+ * LDS-staged device scan.  Replaces the tile_nnz / tile_nnz_acc prefix of
+ * csr2tiledcsr (source/src/main.cu:517-535). */
+size_t bws_row_ptr_workspace_size(int64_t rows);
+bws_status bws_row_ptr(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                       int64_t* row_ptr, void* workspace, size_t workspace_bytes,
+                       bws_stream stream);
+
+/* values[row_ptr[r] + j] = D[r, c_j] where c_j is the column of the j-th set
+ * bit of row r in W.  Replaces the value placement of csr2tiledcsr
+ * (source/src/main.cu:537-563). */
+bws_status bws_pack_values_f32(const float* D, int64_t rows, int64_t cols, int64_t ld,
+                               const uint32_t* W, int64_t ldw, const int64_t* row_ptr,
+                               float* values, bws_stream stream);
+
+/* Bit-matrix transpose: WT (cols x rows bitmap, ldwt >= ceil(rows/32)) has bit
+ * (c, r) set iff W has bit (r, c). */
+bws_status bws_transpose_bitmap(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                                uint32_t* WT, int64_t ldwt, bws_stream stream);
+
+/* Transpose of a value-carrying bitmap: fills AT's words, row_ptr and values
+ * (in AT's (row, column) order).  AT.words/row_ptr/values must point to
+ * writable device buffers of sizes cols*ldwt, cols+1 and nnz. */
+size_t bws_transpose_workspace_size(int64_t rows, int64_t cols);
+bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
+                                int64_t* row_ptr_T, float* values_T, void* workspace,
+                                size_t workspace_bytes, bws_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* (2) Row grouping by bitmask                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Grouping tiles are tile_rows rows x one 32-column word.  Tile index
+ * t = (r / tile_rows) * wpr + w with wpr = ceil(cols/32) (row-tile major,
+ * as in source/src/main.cu:69); tiles = ceil(rows/tile_rows) * wpr.
+ * Inside each tile the rows are assigned in ascending order by first fit:
+ *   mask == 0              -> group_id 0, not a member;
+ *   first g with GM[g]&m==0 -> group_id g, GM[g] |= m, member_row[g][i] = row
+ *                              for each set bit i, group_vals[g][i] = value;
+ *   no such g              -> group_id -1, spilled: slot = spill_cnt++,
+ *                              spill_nnz += popcount(m), spill_table[slot] = row,
+ *                              spill_rev[row] = slot.
+ * This is generate_groups_large (source/src/main.cu:50-247) with its CAS race
+ * resolved in ascending row order and its overflow path made the spill path of
+ * source/src/main.cu:153-166.  Rows are within-tile indices.  Unused
+ * spill_table slots are 0 and non-spilled spill_rev entries -1 (their
+ * initial values in the reference, main.cu:83-84); uncovered group_vals are
+ * 0 and uncovered member_row entries -1. */
+typedef struct {
+  int32_t* group_id;      /* [rows x ldw_g], ldw_g = ldw of the input bitmap      */
+  uint32_t* group_mask;   /* [tiles x G]                                           */
+  float* group_vals;      /* [tiles x G x 32]                                      */
+  int32_t* member_row;    /* [tiles x G x 32] or NULL                              */
+  int32_t* spill_cnt;     /* [tiles]                                               */
+  int32_t* spill_nnz;     /* [tiles]                                               */
+  int32_t* spill_table;   /* [tiles x tile_rows] or NULL                           */
+  int32_t* spill_rev;     /* [tiles x tile_rows] or NULL                           */
+  int64_t* grouped_nnz;   /* [1]: Σ popcount(group_mask), or NULL                  */
+} bws_groups;
+
+/* Limits: 1 <= max_groups <= 32, 1 <= tile_rows <= 65536. */
+int64_t bws_group_tiles(int64_t rows, int64_t cols, int32_t tile_rows);
+bws_status bws_group_rows(const bws_bitmap* A, int32_t tile_rows, int32_t max_groups,
+                          const bws_groups* out, bws_stream stream);
+
+/* Spilled rows as per-tile CSR (generate_spilled_csr, source/src/main.cu:445-496,
+ * with the host prefix of main.cu:2093-2100 done on the device): tile t's
+ * spilled rows in slot order occupy entries spill_row_off[t] ..
+ * spill_row_off[t+1]-1; row_end[e] is the end of that row's nonzeros relative to
+ * spill_nnz_off[t] (the reference's tile_spilled_csrRowPtr); col[] the column
+ * within the word (0..31) and val[] the value (1.0 for a 0/1 mask).
+ * spill_row_off / spill_nnz_off have tiles+1 entries. */
+size_t bws_group_spilled_workspace_size(int64_t rows, int64_t cols, int32_t tile_rows);
+bws_status bws_group_spilled_csr(const bws_bitmap* A, int32_t tile_rows,
+                                 const int32_t* spill_cnt, const int32_t* spill_nnz,
+                                 const int32_t* spill_table, int32_t* spill_row_off,
+                                 int32_t* spill_nnz_off, int32_t* row_end, int32_t* col,
+                                 float* val, void* workspace, size_t workspace_bytes,
+                                 bws_stream stream);
+
+/* Un-grouping (group2dense, source/src/main.cu:829-850): for every (r, w) with
+ * group_id >= 0, D[r, 32w+i] = group_vals[(G*t + g)*32 + i] for each set bit i
+ * of W[r, w]; other elements of D are left untouched. */
+bws_status bws_group_scatter(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                             const int32_t* group_id, const float* group_vals,
+                             int32_t tile_rows, int32_t max_groups, float* D, int64_t ld,
+                             bws_stream stream);
+
+/* Re-grouping (dense2group_from_idx, source/src/main.cu:853-886): the inverse
+ * gather, group_vals[(G*t + g)*32 + i] = D[r, 32w+i]. */
+bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64_t rows,
+                            int64_t cols, int64_t ldw, const int32_t* group_id,
+                            int32_t tile_rows, int32_t max_groups, float* group_vals,
+                            bws_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* (3) Grouped tile GEMM                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Kernel plan.  All-zero (or a NULL plan pointer) selects the built-in
+ * defaults; bws_autotune fills one by measurement. */
+typedef struct {
+  int32_t kernel;        /* 0 = auto, 1 = vector-add rows, 2 = MFMA tiles    */
+  int32_t rows_per_wg;   /* 0 = default                                      */
+  int32_t n_per_wg;      /* 0 = default                                      */
+  int32_t mfma_min_bits; /* MFMA tile when its set bits >= this; 0 = default */
+  int32_t reserved[4];
+} bws_plan;
+
+/* C = alpha * (A · B) + beta * C, A M x K bitmap, B K x N dense (dtype b_dtype,
+ * leading dimension ldb), C M x N FP32.  beta == 0 never reads C.
+ * precision/b_dtype combinations: FP32 with F32 (or exact BF16/F16 input);
+ * BF16 with BF16 or F32 (rounded to nearest even); FP16 with F16 or F32;
+ * BF16X2 with F32.
+ * Semantics of the FP32 add path of backup/layout_trans.cu:145-171 and the
+ * numeric kernel spgemm_compute_1dthread_tcore_v3 (source/src/transform.cuh:
+ * 2062-2248) with the operand roles of a bitmap A and a dense B. */
+size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision,
+                               const bws_plan* plan);
+bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb,
+                    int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                    int32_t precision, const bws_plan* plan, void* workspace,
+                    size_t workspace_bytes, bws_stream stream);
+
+/* Grouped evaluation: C = A·B where each nonzero's value is read from the
+ * group tables (group_vals of its row's group) and spilled rows' values from
+ * A.values (or 1.0).  Equals bws_spmm on the same A (FP32 path), since group
+ * members are disjoint; this is the reference's grouped value use
+ * (transform.cuh:2224-2239) with roles re-assigned. */
+bws_status bws_spmm_groups(const bws_bitmap* A, const int32_t* group_id,
+                           const float* group_vals, int32_t tile_rows, int32_t max_groups,
+                           const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
+                           bws_stream stream);
+
+/* Adjoint dB = alpha * Aᵀ·dC + beta * dB given AT = transpose of A (from
+ * bws_transpose_bitmap / bws_transpose_values).  Identical to bws_spmm on AT. */
+bws_status bws_spmm_tA(const bws_bitmap* AT, const void* dC, int32_t dc_dtype, int64_t lddc,
+                       int64_t N, float* dB, int64_t lddb, float alpha, float beta,
+                       int32_t precision, const bws_plan* plan, void* workspace,
+                       size_t workspace_bytes, bws_stream stream);
+
+/* SDDMM adjoint on the bitmap pattern: for the j-th set bit (r, k) of row r,
+ * dA[row_ptr[r] + j] = Σ_n dC[r, n] · B[k, n].  A.row_ptr is required; A.values
+ * is ignored.  dC and B share dtype `dtype` (F32 for FP32, BF16/F16 or F32 for
+ * the 16-bit paths, F32 for BF16X2). */
+bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
+                     int64_t lddc, int64_t ldb, int64_t N, float* dA_values,
+                     int32_t precision, bws_stream stream);
+
+/* Grouped launch over many bitmap problems in one kernel launch. */
+typedef struct {
+  bws_bitmap A;      /* M x K                                     */
+  const void* B;     /* K x N, dtype of the launch                */
+  int64_t ldb;
+  int64_t N;
+  float* C;          /* M x N FP32                                */
+  int64_t ldc;
+} bws_problem;
+
+size_t bws_spmm_batched_workspace_size(int32_t count);
+/* problems is a HOST array (count entries); the library packs it into the
+ * workspace with one hipMemcpyAsync, so capture the call after the problem
+ * table is final.  C = A·B for every problem (alpha = 1, beta = 0). */
+bws_status bws_spmm_batched(const bws_problem* problems, int32_t count, int32_t b_dtype,
+                            int32_t precision, const bws_plan* plan, void* workspace,
+                            size_t workspace_bytes, bws_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side deterministic generators (no GPU; xoshiro256** seeded by        */
+/* splitmix64).  Replace the reference's glibc rand() fill_random            */
+/* (source/src/utils.h:14-22).                                               */
+/* ------------------------------------------------------------------------ */
+
+/* iid Bernoulli(p) bitmap, rows x cols, row-major draw order. */
+bws_status bws_gen_bitmap_iid(uint64_t seed, int64_t rows, int64_t cols, double p,
+                              uint32_t* W, int64_t ldw);
+/* Clustered bitmap: n_proto prototypes ~ iid Bernoulli(p_proto) over cols; each
+ * row copies prototype c ~ U{0..n_proto-1}, flips every bit with probability
+ * flip, then the rows are shuffled by a uniform random permutation. */
+bws_status bws_gen_bitmap_clustered(uint64_t seed, int64_t rows, int64_t cols,
+                                    int32_t n_proto, double p_proto, double flip,
+                                    uint32_t* W, int64_t ldw);
+/* FP32 fills: kind 0 = N(a, b^2), 1 = U[a, b), 2 = random sign * 10^U[a, b). */
+bws_status bws_gen_f32(uint64_t seed, int64_t n, int32_t kind, double a, double b,
+                       float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BWS_H */

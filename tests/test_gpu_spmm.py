@@ -1,0 +1,207 @@
+"""GPU parity of the grouped tile GEMM family against the FP64 oracle.
+
+Tolerances (error metric |c - r| / Σ_k |a_ik||b_kj|, the componentwise
+condition scale, against the FP64 oracle):
+  FP32 path                          <= 1e-5   (also vs the FP32 restatement)
+  BF16 / FP16 MFMA on rounded B      <= 1e-5   (kernel mechanics: exact 0/1 A, FP32 accumulate)
+  BF16 vs the unrounded FP32 B       <= 1e-2   (BF16 rounding of B: 2^-9)
+  FP16 vs the unrounded FP32 B       <= 2e-3   (2^-11)
+  BF16X2 vs the unrounded FP32 B     <= 1e-5   (hi+lo split: 2^-17)
+"""
+import numpy as np
+import pytest
+import torch
+
+from _helpers import make_problem, rel_err, to_dev
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1, 1, 1.0), (64, 64, 64, 0.1), (100, 70, 33, 0.2), (257, 513, 300, 0.05),
+          (1024, 1024, 1024, 0.1), (512, 2048, 96, 0.01), (31, 1000, 1, 0.5), (300, 32, 1000, 0.3)]
+
+
+@pytest.mark.parametrize("M,K,N,p", SHAPES)
+@pytest.mark.parametrize("values", [False, True])
+def test_spmm_fp32(oracle, bws, dev, M, K, N, p, values):
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M + K, values=values)
+    C = bws.spmm(A, Bd, "fp32").cpu().numpy()
+    e = rel_err(oracle, C, W, K, B, rp, vals)
+    assert e["max_rel_s"] <= 1e-5, e
+    C32 = oracle.spmm_f32_seq(W, K, B, rp, vals)
+    S = oracle.spmm_f64(W, K, B, rp, vals, absolute=True)
+    assert np.all(np.abs(C - C32) <= 2e-5 * S + 1e-30)
+
+
+def test_spmm_fp32_c1_config(oracle, bws, dev):
+    """1024^3, iid Bernoulli(0.1) 0/1 A, B ~ U[-1,1], seed 0."""
+    M = N = K = 1024
+    W = bws.gen_bitmap_iid(0, M, K, 0.1)
+    B = bws.gen_f32(1, (K, N), "uniform", -1.0, 1.0)
+    A = bws.Bitmap(to_dev(W, dev), M, K)
+    C = bws.spmm(A, to_dev(B, dev)).cpu().numpy()
+    e = rel_err(oracle, C, W, K, B)
+    assert e["max_rel_s"] <= 1e-5, e
+    # the reference's own CPU check loop on the dense 0/1 matrix, sampled rows
+    Ad = oracle.unpack_bitmap(W, K)
+    Cref = oracle.check_dense_f32(Ad, B, 0, 64, threads=4)
+    S = oracle.spmm_f64(W[:64], K, B, absolute=True)
+    assert np.all(np.abs(C[:64] - Cref[:64]) <= 2e-5 * S)
+
+
+def test_spmm_alpha_beta_strides(oracle, bws, dev, rng):
+    M, K, N = 200, 300, 150
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=5)
+    C0 = rng.standard_normal((M, N + 7)).astype(np.float32)
+    Cd = to_dev(C0, dev)
+    out = Cd[:, 3:3 + N]
+    bws.spmm(A, Bd, "fp32", alpha=0.5, beta=-2.0, out=out)
+    R = oracle.spmm_f64(W, K, B)
+    expect = 0.5 * R - 2.0 * C0[:, 3:3 + N].astype(np.float64)
+    got = Cd.cpu().numpy()
+    np.testing.assert_allclose(got[:, 3:3 + N], expect, rtol=1e-5, atol=1e-4)
+    assert np.array_equal(got[:, :3], C0[:, :3]) and np.array_equal(got[:, 3 + N:], C0[:, 3 + N:])
+    # strided B (ldb > N)
+    Bbig = to_dev(np.pad(B, ((0, 0), (0, 5))), dev)
+    C2 = bws.spmm(A, Bbig[:, :N]).cpu().numpy()
+    np.testing.assert_allclose(C2, R, rtol=1e-5, atol=1e-5)
+
+
+def test_spmm_k_zero_and_empty(bws, dev):
+    A = bws.Bitmap(torch.zeros((5, 1), dtype=torch.int32, device=dev), 5, 0)
+    B = torch.zeros((0, 7), device=dev)
+    out = torch.full((5, 7), 3.0, device=dev)
+    bws.spmm(A, B, beta=0.5, out=out)
+    assert torch.all(out == 1.5)
+    bws.spmm(A, B, out=out)
+    assert torch.all(out == 0)
+    Z = bws.Bitmap(torch.zeros((64, 2), dtype=torch.int32, device=dev), 64, 64)
+    C = bws.spmm(Z, torch.randn(64, 40, device=dev))
+    assert torch.all(C == 0)
+
+
+@pytest.mark.parametrize("dt", [torch.bfloat16, torch.float16])
+def test_spmm_fp32_with_16bit_input(oracle, bws, dev, dt):
+    M, K, N = 300, 200, 129
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=9)
+    B16 = Bd.to(dt)
+    Bx = B16.float().cpu().numpy()
+    C = bws.spmm(A, B16, "fp32").cpu().numpy()
+    assert rel_err(oracle, C, W, K, Bx)["max_rel_s"] <= 1e-5
+
+
+MFMA_SHAPES = [(128, 128, 128, 0.1), (64, 64, 64, 1.0), (1000, 500, 300, 0.05),
+               (257, 97, 130, 0.3), (1024, 2048, 512, 0.02), (16, 32, 16, 0.5), (5, 3, 2, 0.9)]
+
+
+@pytest.mark.parametrize("M,K,N,p", MFMA_SHAPES)
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x2"])
+def test_spmm_mfma(oracle, bws, dev, M, K, N, p, prec):
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M * 3 + N)
+    plan = bws.make_plan(kernel=2)
+    if prec == "bf16":
+        Bin, Br = Bd.to(torch.bfloat16), oracle.round_bf16(B)
+    elif prec == "fp16":
+        Bin, Br = Bd.to(torch.float16), oracle.round_f16(B)
+    else:
+        Bin, Br = Bd, None
+    C = bws.spmm(A, Bin, prec, plan=plan).cpu().numpy()
+    if Br is not None:
+        # mechanics: exact against the rounded operand
+        assert rel_err(oracle, C, W, K, Br)["max_rel_s"] <= 1e-5
+        bound = 1e-2 if prec == "bf16" else 2e-3
+    else:
+        hi, lo = oracle.split_bf16x2(B)
+        # hi + lo is exact in FP32 (8 + 8 significant bits); mechanics check first
+        assert rel_err(oracle, C, W, K, (hi + lo).astype(np.float32))["max_rel_s"] <= 1e-5
+        bound = 1e-5
+    e = rel_err(oracle, C, W, K, B)
+    assert e["max_rel_s"] <= bound, e
+    # FP32-input 16-bit paths round on the device exactly like the oracle
+    if prec in ("bf16", "fp16"):
+        C2 = bws.spmm(A, Bd, prec, plan=plan).cpu().numpy()
+        assert np.array_equal(C2, C)
+
+
+def test_spmm_groups_equals_spmm(oracle, bws, dev, rng):
+    M, K, N, T, G = 700, 256, 200, 64, 2
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.05, seed=21, values=True)
+    g = bws.group_rows(A, T, G)
+    Cg = bws.spmm_groups(A, g, T, G, Bd).cpu().numpy()
+    e = rel_err(oracle, Cg, W, K, B, rp, vals)
+    assert e["max_rel_s"] <= 1e-5
+    A01 = bws.Bitmap(A.words, M, K)
+    g01 = bws.group_rows(A01, T, G)
+    Cg01 = bws.spmm_groups(A01, g01, T, G, Bd).cpu().numpy()
+    assert rel_err(oracle, Cg01, W, K, B)["max_rel_s"] <= 1e-5
+
+
+@pytest.mark.parametrize("values", [False, True])
+def test_spmm_tA(oracle, bws, dev, values):
+    M, K, N = 600, 500, 130
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=33, values=values)
+    dC = bws.gen_f32(34, (M, N), "normal", 0.0, 1.0)
+    AT = bws.transpose(A)
+    dB = bws.spmm_tA(AT, to_dev(dC, dev)).cpu().numpy()
+    Ad = np.zeros((M, K), np.float64)
+    mask = oracle.unpack_bitmap(W, K) != 0
+    Ad[mask] = vals if values else 1.0
+    R = Ad.T @ dC.astype(np.float64)
+    S = np.abs(Ad).T @ np.abs(dC).astype(np.float64)
+    assert np.max(np.abs(dB - R) / np.maximum(S, 1e-30)) <= 1e-5
+
+
+@pytest.mark.parametrize("M,K,N,p", [(100, 80, 64, 0.1), (300, 500, 1500, 0.05), (17, 33, 3, 0.5)])
+def test_sddmm_fp32(oracle, bws, dev, M, K, N, p):
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=44)
+    dC = bws.gen_f32(45, (M, N), "normal", 0.0, 1.0)
+    A = bws.Bitmap(A.words, M, K, to_dev(rp, dev))
+    dA = bws.sddmm(A, to_dev(dC, dev), Bd).cpu().numpy()
+    R = oracle.sddmm_f64(W, K, rp, dC, B)
+    S = oracle.sddmm_f64(W, K, rp, dC, B, absolute=True)
+    assert np.max(np.abs(dA - R) / np.maximum(S, 1e-30)) <= 1e-5
+
+
+def test_spmm_batched(oracle, bws, dev):
+    probs, refs = [], []
+    for i, (M, K, p) in enumerate([(64, 128, 0.1), (300, 64, 0.2), (1000, 256, 0.01)]):
+        A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, 100, p, seed=100 + i)
+        Cd = torch.empty((M, 100), device=dev)
+        probs.append((A, Bd, Cd))
+        refs.append((W, K, B))
+    bws.spmm_batched(probs)
+    for (A, Bd, Cd), (W, K, B) in zip(probs, refs):
+        assert rel_err(oracle, Cd.cpu().numpy(), W, K, B)["max_rel_s"] <= 1e-5
+
+
+def test_spmm_deterministic(bws, oracle, dev):
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, 1024, 1024, 512, 0.05, seed=77)
+    a = bws.spmm(A, Bd)
+    b = bws.spmm(A, Bd)
+    assert torch.equal(a, b)
+    p = bws.make_plan(kernel=2)
+    a = bws.spmm(A, Bd.to(torch.bfloat16), "bf16", plan=p)
+    b = bws.spmm(A, Bd.to(torch.bfloat16), "bf16", plan=p)
+    assert torch.equal(a, b)
+
+
+def test_spmm_4096_properties(oracle, bws, dev):
+    """Full-size clustered problem (4096^3, ~5% dense): sampled rows against
+    the FP64 oracle, and linearity A·(B1 + B2) = A·B1 + A·B2."""
+    M = N = K = 4096
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.03125, seed=0, clustered=True)
+    C = bws.spmm(A, Bd)
+    rows = np.arange(0, M, 97)
+    Cs = C[rows].cpu().numpy()
+    e = rel_err(oracle, Cs, W[rows], K, B)
+    assert e["max_rel_s"] <= 1e-5, e
+    B2 = torch.randn_like(Bd)
+    lhs = bws.spmm(A, Bd + B2)
+    rhs = C + bws.spmm(A, B2)
+    S = bws.spmm(A, Bd.abs() + B2.abs())
+    assert torch.max((lhs - rhs).abs() / S.clamp_min(1e-30)).item() <= 3e-5
+    # checksum of checksums: C·1 = A·(B·1)
+    ones = torch.ones(N, 1, device=dev)
+    lhs = C.double() @ ones.double()
+    rhs = bws.spmm(A, (Bd.double() @ ones.double()).float().contiguous()).double()
+    Sabs = bws.spmm(A, Bd.abs().sum(1, keepdim=True).contiguous()).double()
+    assert torch.max((lhs - rhs).abs() / Sabs.clamp_min(1e-30)).item() <= 1e-5
