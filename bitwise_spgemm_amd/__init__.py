@@ -71,8 +71,8 @@ class _Groups(C.Structure):
 
 
 class _Plan(C.Structure):
-    _fields_ = [("kernel", C.c_int32), ("rows_per_wg", C.c_int32), ("n_per_wg", C.c_int32),
-                ("mfma_min_bits", C.c_int32), ("reserved", C.c_int32 * 4)]
+    _fields_ = [("kernel", C.c_int32), ("rows_per_wg", C.c_int32), ("waves_per_wg", C.c_int32),
+                ("k_stage", C.c_int32), ("mfma_min_bits", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class _Problem(C.Structure):
@@ -425,8 +425,9 @@ def group_gather(D: torch.Tensor, A: Bitmap, groups: dict, tile_rows: int, max_g
 # ----------------------------------------------------------------------------
 # (3) grouped tile GEMM
 # ----------------------------------------------------------------------------
-def make_plan(kernel: int = 0, rows_per_wg: int = 0, n_per_wg: int = 0, mfma_min_bits: int = 0):
-    return _Plan(kernel, rows_per_wg, n_per_wg, mfma_min_bits, (C.c_int32 * 4)())
+def make_plan(kernel: int = 0, rows_per_wg: int = 0, waves_per_wg: int = 0, k_stage: int = 0,
+              mfma_min_bits: int = 0):
+    return _Plan(kernel, rows_per_wg, waves_per_wg, k_stage, mfma_min_bits, (C.c_int32 * 3)())
 
 
 def spmm(A: Bitmap, B: torch.Tensor, precision: str = "fp32", alpha: float = 1.0,

@@ -44,6 +44,15 @@ bws_status check_bitmap(const bws_bitmap* A, bool need_row_ptr);
 // Maximum grid dimension used for 1-D launches (blocks are looped over).
 constexpr int64_t kMaxGrid = 1 << 30;
 
+// Vector-add path tile configuration (rows of C per workgroup, waves per
+// workgroup, K rows per LDS stage).
+struct ValuCfg {
+  int32_t rows = 256;
+  int32_t waves = 8;
+  int32_t kstage = 64;
+};
+bool valu_cfg_valid(const ValuCfg& c);
+
 // Device scratch carving: a linear allocator over the caller's workspace.
 struct Carve {
   char* base;

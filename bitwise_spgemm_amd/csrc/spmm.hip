@@ -5,8 +5,46 @@
 namespace bws {
 
 bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
-                     float* C, int64_t ldc, float alpha, float beta, int32_t rows_per_wave,
+                     float* C, int64_t ldc, float alpha, float beta, const ValuCfg& cfg,
                      hipStream_t s);
+
+bool valu_cfg_valid(const ValuCfg& c) {
+  static const int32_t ok[][3] = {{256, 8, 64}, {256, 16, 64}, {256, 16, 32}, {128, 8, 64},
+                                  {128, 8, 32},  {64, 8, 32},   {128, 16, 32}, {128, 16, 64},
+                                  {64, 16, 32}};
+  for (const auto& v : ok)
+    if (c.rows == v[0] && c.waves == v[1] && c.kstage == v[2]) return true;
+  return false;
+}
+
+// Default vector-add tile for an M x N output.
+ValuCfg default_valu_cfg(int64_t M, int64_t N) {
+  ValuCfg c;
+  const int64_t tiles256 = ceil_div(M, 256) * ceil_div(N, 256);
+  if (tiles256 >= 192) {  // 256 x 256 tiles fill the 256 CUs
+    c.waves = 16;          // 16 waves x 16 rows: 4 waves per SIMD hide LDS latency
+    return c;
+  }
+  const int64_t tiles128 = ceil_div(M, 128) * ceil_div(N, 256);
+  if (tiles128 >= 192) {
+    c.rows = 128;
+    return c;
+  }
+  c.rows = 64;
+  c.kstage = 32;
+  return c;
+}
+
+// The plan's vector-add tile, or the default where the plan leaves it 0.
+ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N) {
+  ValuCfg c = default_valu_cfg(M, N);
+  if (plan && plan->rows_per_wg) {
+    c.rows = plan->rows_per_wg;
+    c.waves = plan->waves_per_wg ? plan->waves_per_wg : (c.rows == 256 ? 16 : 8);
+    c.kstage = plan->k_stage ? plan->k_stage : (c.rows == 64 ? 32 : 64);
+  }
+  return c;
+}
 bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,
                      hipStream_t s);
@@ -75,12 +113,7 @@ bws_status spmm_dispatch(const bws_bitmap* A, const void* B, int32_t b_dtype, in
     if (A->values) return BWS_EUNSUPPORTED;
     return spmm_mfma(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, s);
   }
-  int32_t rpw = plan && plan->rows_per_wg ? plan->rows_per_wg / 8 : 0;
-  if (rpw == 0) {
-    const int64_t tiles32 = ceil_div(M, 256) * ceil_div(N, 256);
-    rpw = tiles32 >= 192 ? 32 : 8;
-  }
-  return spmm_valu(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, rpw, s);
+  return spmm_valu(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, plan_valu_cfg(plan, M, N), s);
 }
 
 }  // namespace bws
@@ -108,7 +141,8 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   if (N < 0 || !dtype_ok(b_dtype) || !prec_ok(precision)) return BWS_EINVAL;
   if (!combo_ok(precision, b_dtype)) return BWS_EUNSUPPORTED;
   if (plan && (plan->kernel < 0 || plan->kernel > 2 || plan->rows_per_wg < 0 ||
-               (plan->rows_per_wg != 0 && plan->rows_per_wg != 64 && plan->rows_per_wg != 256)))
+               (plan->rows_per_wg != 0 &&
+                !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N)))))
     return BWS_EINVAL;
   if (A->rows == 0 || N == 0) return BWS_OK;
   if (!C || ldc < N) return BWS_EINVAL;

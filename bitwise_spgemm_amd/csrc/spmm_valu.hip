@@ -210,6 +210,235 @@ k_spmm_rows(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   }
 }
 
+// Version 2 of the FP32 row kernel, for FP32 B with ldb % 4 == 0, N % 4 == 0
+// and 16-byte aligned B and C.  Differences from k_spmm_rows:
+//  - a stage covers 64 K rows (two bitmap words); each row's 64-bit mask
+//    drives one bit loop per stage;
+//  - B is staged with global_load_lds_dwordx4 (HBM -> LDS directly, 1 KB per
+//    wave instruction = one 256-column row segment), issued one stage ahead,
+//    so no registers hold staged data and the loads overlap the bit loops.
+//    The loads are issued from a two-instruction asm block (they write LDS
+//    only, no VGPR results), and the one wait on them (vmcnt(0)) sits right
+//    before each stage's barrier, so nothing waits on the prefetch mid-stage;
+//  - bitmap words are prefetched two stages ahead and values one stage ahead,
+//    so every global load a stage consumes was issued before its barrier;
+//  - set bits are consumed in batches of 4, 2 and 1 independent LDS reads,
+//    keeping several reads in flight per wave; within a row the adds stay in
+//    ascending k order.
+// Lanes beyond N or rows beyond K load a clamped, valid address; that data is
+// never used (rows beyond K have no bits, columns beyond N are never stored).
+__device__ __forceinline__ void lds_dma16(const void* g, uint32_t lds_addr) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(g), "s"(lds_addr)
+               : "memory", "m0");
+}
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ uint32_t lds_addr_of(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// m holds the stage's row mask bit-reversed (bit k <-> stage column k), so the
+// lowest set bit is the next column: s_ff1 + shift + xor on the scalar unit.
+__device__ __forceinline__ void take_bit(uint64_t& m, int& k) {
+  k = __builtin_ctzll(m);
+  m ^= 1ull << k;
+}
+
+template <int WAVES, int RPW, int KS, bool VALS>
+__global__ void __launch_bounds__(WAVES * 64)
+k_spmm_rows2(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+             const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
+             const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
+             int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
+             int64_t per_xcd) {
+  static_assert(KS == 32 || KS == 64, "stage is one or two bitmap words");
+  constexpr int WPS = KS / 32;  // bitmap words per stage
+  __shared__ float4 sB0[KS][64];
+  __shared__ float4 sB1[KS][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
+  const int64_t n0 = cb * kBN;
+  const int64_t mw = rb * (WAVES * RPW) + (int64_t)wave * RPW;
+  const int64_t nst = (K + KS - 1) / KS;
+  const int64_t wpr = (K + 31) / 32;
+  // lane l < RPW holds word WPS*s of row mw+l, lane 32+l holds word 2s+1 (KS = 64)
+  const int rl = lane & 31;
+  const bool row_lane = rl < RPW && mw + rl < M;
+  const int64_t wrow = row_lane ? (mw + rl) * ldw : 0;
+  const int64_t ncol = (n0 + 4 * lane + 4 <= N) ? n0 + 4 * lane : N - 4;  // clamped column
+  const float* bcol = B + ncol;
+  const uint32_t base0 = lds_addr_of(&sB0[0][0]), base1 = lds_addr_of(&sB1[0][0]);
+
+  float4 acc[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  auto issue = [&](int64_t st, uint32_t base) {
+#pragma unroll
+    for (int kr = 0; kr < KS / WAVES; ++kr) {
+      const int row = wave + WAVES * kr;
+      int64_t k = st * KS + row;
+      k = k < K ? k : K - 1;
+      lds_dma16(bcol + k * ldb, base + (uint32_t)row * 1024u);
+    }
+  };
+  auto load_words = [&](int64_t st) -> uint32_t {
+    const int64_t w = WPS * st + (lane >> 5);
+    const bool ok = WPS == 2 || lane < 32;
+    return (ok && row_lane && st < nst && w < wpr) ? W[wrow + w] : 0u;
+  };
+  auto stage_mask = [&](int64_t st) -> uint64_t {
+    const int64_t kvalid = K - st * KS;
+    return kvalid >= 64 ? ~0ull : (~0ull << (64 - kvalid));
+  };
+  // this lane's row mask (lane l < RPW <-> row l) for the stage words w
+  auto lane_row_mask = [&](uint32_t w, uint64_t smask) -> uint64_t {
+    const uint32_t w_hi = (uint32_t)__shfl((int)w, rl, 64);
+    const uint32_t w_lo = WPS == 2 ? (uint32_t)__shfl((int)w, rl + 32, 64) : 0u;
+    return ((((uint64_t)w_hi) << 32) | w_lo) & smask;
+  };
+
+  int64_t e_l = 0;         // VALS: lane l < RPW: value offset of row l at the current stage
+  int cnt_cur = 0;         // VALS: set bits of row l in the current stage
+  float vv[VALS ? RPW : 1];
+  if (VALS) e_l = row_lane ? row_ptr[mw + rl] : 0;
+
+  issue(0, base0);
+  uint32_t wcur = load_words(0);
+  uint32_t wnext = load_words(1);
+  if (VALS) {
+    cnt_cur = __popcll(lane_row_mask(wcur, stage_mask(0)));
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int64_t e = readlane_i64(e_l, r);
+      const int c = __builtin_amdgcn_readlane(cnt_cur, r);
+      vv[r] = lane < c ? values[e + lane] : 0.0f;
+    }
+  }
+
+  auto stage = [&](int64_t st, float4 (*cur)[64], uint32_t nxt_base) {
+    wait_vmem();     // this wave's loads for stage st (and its words/values) landed
+    // The registers below were loaded in earlier stages and have now landed;
+    // re-defining them through an empty asm statement tells the compiler so,
+    // which keeps it from waiting (vmcnt(0)) on the prefetch issued next.
+    asm volatile("" : "+v"(wcur), "+v"(wnext));
+    if (VALS) {
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) asm volatile("" : "+v"(vv[VALS ? r : 0]));
+    }
+    __syncthreads(); // everyone's; the other buffer is no longer being read
+    const bool more = st + 1 < nst;
+    if (more) issue(st + 1, nxt_base);
+    const uint32_t wn2 = load_words(st + 2);
+    const uint64_t smask = stage_mask(st);
+    float vn[VALS ? RPW : 1];
+    int cnt_next = 0;
+    if (VALS) {
+      const int64_t e_next = e_l + cnt_cur;
+      cnt_next = more ? __popcll(lane_row_mask(wnext, stage_mask(st + 1))) : 0;
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        const int64_t e = readlane_i64(e_next, r);
+        const int c = __builtin_amdgcn_readlane(cnt_next, r);
+        vn[r] = lane < c ? values[e + lane] : 0.0f;
+      }
+    }
+
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)wcur, r);
+      const uint32_t lo =
+          WPS == 2 ? (uint32_t)__builtin_amdgcn_readlane((int)wcur, r + 32) : 0u;
+      uint64_t m = __builtin_bitreverse64(((((uint64_t)hi) << 32) | lo) & smask);
+      int n = __popcll(m);
+      int j = 0;
+      while (n >= 4) {
+        int k0, k1, k2, k3;
+        take_bit(m, k0);
+        take_bit(m, k1);
+        take_bit(m, k2);
+        take_bit(m, k3);
+        const float4 b0 = cur[k0][lane], b1 = cur[k1][lane], b2 = cur[k2][lane],
+                     b3 = cur[k3][lane];
+        if (VALS) {
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b0);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 1), b1);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 2), b2);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 3), b3);
+        } else {
+          add4(acc[r], b0);
+          add4(acc[r], b1);
+          add4(acc[r], b2);
+          add4(acc[r], b3);
+        }
+        n -= 4;
+        j += 4;
+      }
+      if (n >= 2) {
+        int k0, k1;
+        take_bit(m, k0);
+        take_bit(m, k1);
+        const float4 b0 = cur[k0][lane], b1 = cur[k1][lane];
+        if (VALS) {
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b0);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 1), b1);
+        } else {
+          add4(acc[r], b0);
+          add4(acc[r], b1);
+        }
+        n -= 2;
+        j += 2;
+      }
+      if (n) {
+        int k0;
+        take_bit(m, k0);
+        const float4 b0 = cur[k0][lane];
+        if (VALS) fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b0);
+        else add4(acc[r], b0);
+      }
+    }
+    if (VALS) {
+      e_l += cnt_cur;
+      cnt_cur = cnt_next;
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) vv[r] = vn[r];
+    }
+    wcur = wnext;
+    wnext = wn2;
+  };
+
+  for (int64_t st = 0; st < nst; st += 2) {
+    stage(st, sB0, base1);
+    if (st + 1 < nst) stage(st + 1, sB1, base0);
+  }
+
+  const int64_t n = n0 + 4 * lane;
+  if (n >= N) return;
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int64_t row = mw + r;
+    if (row >= M) break;
+    float4* c = reinterpret_cast<float4*>(C + row * ldc + n);
+    float4 v = acc[r];
+    v.x *= alpha;
+    v.y *= alpha;
+    v.z *= alpha;
+    v.w *= alpha;
+    if (beta != 0.0f) {
+      const float4 o = *c;
+      v.x = fmaf(beta, o.x, v.x);
+      v.y = fmaf(beta, o.y, v.y);
+      v.z = fmaf(beta, o.z, v.z);
+      v.w = fmaf(beta, o.w, v.w);
+    }
+    *c = v;
+  }
+}
+
 // Grouped evaluation: one wave per (row, 256-column block), values from the
 // group tables (group_vals of the row's group in this word) or, for a spilled
 // row, from A's own values.
@@ -295,6 +524,20 @@ k_sddmm_f32(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   }
 }
 
+template <int WAVES, int RPW, int KS, bool VALS>
+bws_status launch_rows2(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
+                        int64_t ldc, float alpha, float beta, hipStream_t s) {
+  const int64_t row_blocks = ceil_div(A->rows, WAVES * RPW);
+  const int64_t col_blocks = ceil_div(N, kBN);
+  const int64_t tiles = row_blocks * col_blocks;
+  const int64_t per_xcd = ceil_div(tiles, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  hipLaunchKernelGGL((k_spmm_rows2<WAVES, RPW, KS, VALS>), dim3((unsigned)(8 * per_xcd)),
+                     dim3(WAVES * 64), 0, s, A->words, A->rows, A->cols, A->ldw, A->row_ptr,
+                     A->values, B, ldb, N, C, ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
+  return launch_check();
+}
+
 template <int TB, int RPW, bool VALS, bool VEC>
 bws_status launch_rows(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, float* C,
                        int64_t ldc, float alpha, float beta, hipStream_t s) {
@@ -333,13 +576,32 @@ inline int grid_for(int64_t items, int64_t per_block) {
 
 // Vector-add path launcher shared with the dispatcher in spmm.cpp.
 bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
-                     float* C, int64_t ldc, float alpha, float beta, int32_t rows_per_wave,
+                     float* C, int64_t ldc, float alpha, float beta, const ValuCfg& cfg,
                      hipStream_t s) {
   const size_t esz = b_dtype == BWS_DTYPE_F32 ? 4 : 2;
   const bool vec = (ldb % 4 == 0) && (ldc % 4 == 0) &&
                    (reinterpret_cast<uintptr_t>(B) % (4 * esz) == 0) &&
                    (reinterpret_cast<uintptr_t>(C) % 16 == 0);
-  const bool big = rows_per_wave >= 32;
+  if (b_dtype == BWS_DTYPE_F32 && vec && N % 4 == 0 && N >= 4) {
+    const float* Bf = static_cast<const float*>(B);
+    const bool vals = A->values != nullptr;
+#define BWS_V2(WV, RP, KS)                                                              \
+  if (cfg.waves == WV && cfg.rows == WV * RP && cfg.kstage == KS)                       \
+    return vals ? launch_rows2<WV, RP, KS, true>(A, Bf, ldb, N, C, ldc, alpha, beta, s) \
+                : launch_rows2<WV, RP, KS, false>(A, Bf, ldb, N, C, ldc, alpha, beta, s);
+    BWS_V2(8, 32, 64)
+    BWS_V2(16, 16, 64)
+    BWS_V2(16, 16, 32)
+    BWS_V2(8, 16, 64)
+    BWS_V2(8, 16, 32)
+    BWS_V2(8, 8, 32)
+    BWS_V2(16, 8, 32)
+    BWS_V2(16, 8, 64)
+    BWS_V2(16, 4, 32)
+#undef BWS_V2
+    return BWS_EINVAL;
+  }
+  const bool big = cfg.rows >= 256;
 #define BWS_ROWS(TB)                                                                   \
   return big ? launch_rows_v<TB, 32>(A, B, ldb, N, C, ldc, alpha, beta, vec, s)        \
              : launch_rows_v<TB, 8>(A, B, ldb, N, C, ldc, alpha, beta, vec, s)

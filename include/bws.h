@@ -235,13 +235,20 @@ bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64
 /* ------------------------------------------------------------------------ */
 
 /* Kernel plan.  All-zero (or a NULL plan pointer) selects the built-in
- * defaults; bws_autotune fills one by measurement. */
+ * defaults; bws_autotune fills one by measurement.
+ *   kernel        0 = auto, 1 = vector-add rows (FP32 accumulate of set bits),
+ *                 2 = MFMA tiles (16-bit precisions only)
+ *   rows_per_wg   vector-add path: rows of C per workgroup (64, 128 or 256)
+ *   waves_per_wg  vector-add path: 8 or 16
+ *   k_stage       vector-add path: K rows staged per LDS buffer (32 or 64)
+ *   mfma_min_bits reserved for the density classifier (0 = default) */
 typedef struct {
-  int32_t kernel;        /* 0 = auto, 1 = vector-add rows, 2 = MFMA tiles    */
-  int32_t rows_per_wg;   /* 0 = default                                      */
-  int32_t n_per_wg;      /* 0 = default                                      */
-  int32_t mfma_min_bits; /* MFMA tile when its set bits >= this; 0 = default */
-  int32_t reserved[4];
+  int32_t kernel;
+  int32_t rows_per_wg;
+  int32_t waves_per_wg;
+  int32_t k_stage;
+  int32_t mfma_min_bits;
+  int32_t reserved[3];
 } bws_plan;
 
 /* C = alpha * (A · B) + beta * C, A M x K bitmap, B K x N dense (dtype b_dtype,
