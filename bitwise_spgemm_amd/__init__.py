@@ -45,6 +45,7 @@ BWS_EUNSUPPORTED = -3
 BWS_EHIP = -4
 BWS_EWORKSPACE = -5
 BWS_ENODEVICE = -6
+BWS_ENOTFOUND = -7
 
 DTYPE = {torch.float32: 0, torch.bfloat16: 1, torch.float16: 2}
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x2": 3}
@@ -77,7 +78,22 @@ class _Plan(C.Structure):
 
 class _Problem(C.Structure):
     _fields_ = [("A", _Bitmap), ("B", C.c_void_p), ("ldb", C.c_int64), ("N", C.c_int64),
-                ("C", C.c_void_p), ("ldc", C.c_int64)]
+                ("C", C.c_void_p), ("ldc", C.c_int64), ("nnz_hint", C.c_int64)]
+
+
+BATCH_MAX_LAUNCH = 16
+
+
+class _BatchLaunch(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("kind", "rows_per_wg", "waves_per_wg", "k_stage",
+                                        "values", "vec", "first", "count")] + \
+               [("total_tiles", C.c_int64)]
+
+
+class _Batch(C.Structure):
+    _fields_ = [("count", C.c_int32), ("b_dtype", C.c_int32), ("precision", C.c_int32),
+                ("n_launch", C.c_int32), ("launch", _BatchLaunch * BATCH_MAX_LAUNCH),
+                ("workspace", C.c_void_p), ("problems", C.c_void_p), ("plan", _Plan)]
 
 
 _P, _I64, _I32, _U64, _F, _D, _SZ = (C.c_void_p, C.c_int64, C.c_int32, C.c_uint64, C.c_float,
@@ -113,10 +129,24 @@ _SIGS = {
     "bws_spmm_groups": ([_BP, _P, _P, _I32, _I32, _P, _I64, _I64, _P, _I64, _P], _I32),
     "bws_spmm_tA": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P,
                      _SZ, _P], _I32),
-    "bws_sddmm": ([_BP, _P, _P, _I32, _I64, _I64, _I64, _P, _I32, _P], _I32),
+    "bws_sddmm_workspace_size": ([_BP], _SZ),
+    "bws_sddmm": ([_BP, _P, _P, _I32, _I64, _I64, _I64, _P, _I32, _P, _SZ, _P], _I32),
     "bws_spmm_batched_workspace_size": ([_I32], _SZ),
     "bws_spmm_batched": ([C.POINTER(_Problem), _I32, _I32, _I32, C.POINTER(_Plan), _P, _SZ, _P],
                          _I32),
+    "bws_batch_workspace_size": ([_I32], _SZ),
+    "bws_batch_prepare": ([C.POINTER(_Problem), _I32, _I32, _I32, C.POINTER(_Plan), _P, _SZ,
+                           C.POINTER(_Batch)], _I32),
+    "bws_batch_run": ([C.POINTER(_Batch), _P], _I32),
+    "bws_autotune": ([_BP, _I64, _P, _I32, _I64, _I64, _P, _I64, _I32, _I32, C.POINTER(_Plan),
+                      C.POINTER(C.c_float), _P], _I32),
+    "bws_autotune_lookup": ([_I64, _I64, _I64, _I64, _I32, C.POINTER(_Plan)], _I32),
+    "bws_autotune_clear": ([], _I32),
+    "bws_autotune_save": ([C.c_char_p], _I32),
+    "bws_autotune_load": ([C.c_char_p], _I32),
+    "bws_symbolic_or": ([_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_spmm_bitplanes_i8": ([_P, _I64, _I64, _I64, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32,
+                               C.POINTER(_Plan), _P], _I32),
     "bws_gen_bitmap_iid": ([_U64, _I64, _I64, _D, _P, _I64], _I32),
     "bws_gen_bitmap_clustered": ([_U64, _I64, _I64, _I32, _D, _D, _P, _I64], _I32),
     "bws_gen_f32": ([_U64, _I64, _I32, _D, _D, _P], _I32),
@@ -477,37 +507,109 @@ def spmm_groups(A: Bitmap, groups: dict, tile_rows: int, max_groups: int,
     return out
 
 
-def sddmm(A: Bitmap, dC: torch.Tensor, B: torch.Tensor, precision: str = "fp32") -> torch.Tensor:
+def sddmm(A: Bitmap, dC: torch.Tensor, B: torch.Tensor, precision: str = "fp32",
+          out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """dA values = (dC·Bᵀ) at A's set bits, in A's value order."""
     _require_gpu(dC, B)
+    assert dC.dtype == B.dtype
     rp = A.row_ptr if A.row_ptr is not None else row_ptr(A)
-    nnz = int(rp[-1].item())
-    out = torch.empty(max(nnz, 1), dtype=torch.float32, device=B.device)
+    nnz = int(rp[-1].item()) if out is None else out.numel()
+    if out is None:
+        out = torch.empty(max(nnz, 1), dtype=torch.float32, device=B.device)[:nnz]
     bm = Bitmap(A.words, A.rows, A.cols, rp, None)
     s = bm.struct()
+    ws = _workspace(lib().bws_sddmm_workspace_size(C.byref(s)), B.device)
     _check(lib().bws_sddmm(C.byref(s), _ptr(dC), _ptr(B), DTYPE[B.dtype], dC.stride(0),
-                           B.stride(0), B.shape[1], _ptr(out), PRECISION[precision],
-                           _stream(B.device)), "bws_sddmm")
-    return out[:nnz]
+                           B.stride(0), B.shape[1], _ptr(out) if nnz else _ptr(ws),
+                           PRECISION[precision], _ptr(ws), ws.numel(), _stream(B.device)),
+           "bws_sddmm")
+    return out
 
 
-def spmm_batched(problems, precision: str = "fp32", plan=None):
-    """problems: list of (A: Bitmap, B: Tensor, C: Tensor); C = A·B for each."""
-    n = len(problems)
-    arr = (_Problem * max(n, 1))()
-    dtype = None
-    dev = None
-    for i, (A, B, Cm) in enumerate(problems):
-        arr[i] = _Problem(A.struct(), B.data_ptr(), B.stride(0), B.shape[1], Cm.data_ptr(),
-                          Cm.stride(0))
-        dtype = DTYPE[B.dtype]
-        dev = B.device
-    if n == 0:
-        return
-    ws = _workspace(lib().bws_spmm_batched_workspace_size(n), dev)
-    _check(lib().bws_spmm_batched(arr, n, dtype, PRECISION[precision],
-                                  None if plan is None else C.byref(plan), _ptr(ws), ws.numel(),
-                                  _stream(dev)), "bws_spmm_batched")
+class Batch:
+    """A prepared grouped launch (bws_batch_prepare); run() is capturable."""
+
+    def __init__(self, problems, precision="fp32", plan=None, nnz_hints=None):
+        n = len(problems)
+        self.keep = problems                      # device tensors must outlive the batch
+        self.arr = (_Problem * max(n, 1))()
+        self.dtype = DTYPE[problems[0][1].dtype] if n else 0
+        self.device = problems[0][1].device if n else torch.device("cuda")
+        for i, (A, B, Cm) in enumerate(problems):
+            hint = 0 if nnz_hints is None else int(nnz_hints[i])
+            self.arr[i] = _Problem(A.struct(), B.data_ptr(), B.stride(0), B.shape[1],
+                                   Cm.data_ptr(), Cm.stride(0), hint)
+        self.ws = _workspace(lib().bws_batch_workspace_size(n), self.device)
+        self.batch = _Batch()
+        _check(lib().bws_batch_prepare(self.arr, n, self.dtype, PRECISION[precision],
+                                       None if plan is None else C.byref(plan), _ptr(self.ws),
+                                       self.ws.numel(), C.byref(self.batch)), "bws_batch_prepare")
+
+    @property
+    def launches(self):
+        return [dict(kind=L.kind, rows=L.rows_per_wg, waves=L.waves_per_wg, k_stage=L.k_stage,
+                     values=L.values, count=L.count, tiles=L.total_tiles)
+                for L in self.batch.launch[:self.batch.n_launch]]
+
+    def run(self):
+        _check(lib().bws_batch_run(C.byref(self.batch), _stream(self.device)), "bws_batch_run")
+
+
+def spmm_batched(problems, precision: str = "fp32", plan=None, nnz_hints=None):
+    """problems: list of (A: Bitmap, B: Tensor, C: Tensor); C = A·B for each (one
+    grouped launch per tile configuration)."""
+    if not problems:
+        return None
+    b = Batch(problems, precision, plan, nnz_hints)
+    b.run()
+    return b
+
+
+def autotune(A: Bitmap, B: torch.Tensor, precision: str = "fp32", nnz: Optional[int] = None,
+             repeats: int = 3):
+    """Time the candidate plans on this problem; returns (plan, ms) and caches it."""
+    nnz = A.nnz if nnz is None else nnz
+    N = B.shape[1]
+    out = torch.empty((A.rows, N), dtype=torch.float32, device=B.device)
+    plan = _Plan()
+    ms = C.c_float()
+    s = A.struct()
+    _check(lib().bws_autotune(C.byref(s), nnz, _ptr(B), DTYPE[B.dtype], B.stride(0), N, _ptr(out),
+                              out.stride(0), PRECISION[precision], repeats, C.byref(plan),
+                              C.byref(ms), _stream(B.device)), "bws_autotune")
+    return plan, ms.value
+
+
+def autotune_lookup(M, K, N, nnz, precision="fp32"):
+    plan = _Plan()
+    st = lib().bws_autotune_lookup(M, K, N, nnz, PRECISION[precision], C.byref(plan))
+    return plan if st == BWS_OK else None
+
+
+def symbolic_or(A: Bitmap, Bm: Bitmap) -> Bitmap:
+    """pre_spgemm's symbolic phase: the bitmap of A·B's structural nonzeros."""
+    assert A.cols == Bm.rows
+    dev = A.words.device
+    ldwc = words_for(Bm.cols)
+    WC = torch.empty((A.rows, max(ldwc, 1)), dtype=torch.int32, device=dev)
+    _check(lib().bws_symbolic_or(_ptr(A.words), A.rows, A.cols, A.ldw, _ptr(Bm.words), Bm.cols,
+                                 Bm.ldw, _ptr(WC), WC.shape[1], _stream(dev)), "bws_symbolic_or")
+    return Bitmap(WC, A.rows, Bm.cols)
+
+
+def spmm_bitplanes_i8(P: torch.Tensor, cols: int, B: torch.Tensor, precision: str = "fp32",
+                      alpha: float = 1.0, beta: float = 0.0, out=None, plan=None) -> torch.Tensor:
+    """C = Σ_b w_b P_b·B for int8 bit-planes P [8, M, ldw] (weights 1..64, -128)."""
+    _, M, ldw = P.shape
+    N = B.shape[1]
+    if out is None:
+        out = torch.empty((M, N), dtype=torch.float32, device=B.device)
+    _check(lib().bws_spmm_bitplanes_i8(_ptr(P), M, cols, ldw, _ptr(B), DTYPE[B.dtype], B.stride(0),
+                                       N, _ptr(out), out.stride(0), alpha, beta,
+                                       PRECISION[precision],
+                                       None if plan is None else C.byref(plan),
+                                       _stream(B.device)), "bws_spmm_bitplanes_i8")
+    return out
 
 
 # ----------------------------------------------------------------------------

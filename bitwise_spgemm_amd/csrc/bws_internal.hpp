@@ -53,6 +53,17 @@ struct ValuCfg {
 };
 bool valu_cfg_valid(const ValuCfg& c);
 
+// One problem of a grouped launch, as the kernels read it from the workspace.
+struct BatchEntry {
+  const uint32_t* W;
+  const int64_t* row_ptr;
+  const float* values;
+  const void* B;
+  float* C;
+  int64_t M, K, ldw, ldb, N, ldc;
+  int64_t row_blocks, col_blocks;
+};
+
 // Device scratch carving: a linear allocator over the caller's workspace.
 struct Carve {
   char* base;

@@ -174,38 +174,26 @@ bws_status bws_spmm_groups(const bws_bitmap* A, const int32_t* group_id, const f
                             stream);
 }
 
+size_t bws_sddmm_workspace_size(const bws_bitmap* A) {
+  if (!A || A->rows < 0 || A->cols < 0) return 0;
+  return sddmm_mfma_workspace(A);
+}
+
 bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
                      int64_t lddc, int64_t ldb, int64_t N, float* dA_values, int32_t precision,
-                     bws_stream stream) {
+                     void* workspace, size_t workspace_bytes, bws_stream stream) {
   BWS_TRY(check_bitmap(A, true));
   if (N < 0 || !dtype_ok(dtype) || !prec_ok(precision)) return BWS_EINVAL;
+  if (precision == BWS_PREC_FP32 && dtype != BWS_DTYPE_F32) return BWS_EUNSUPPORTED;
+  if (!combo_ok(precision, dtype)) return BWS_EUNSUPPORTED;
   if (A->rows == 0 || A->cols == 0) return BWS_OK;
   if (!dA_values) return BWS_EINVAL;
   if (N > 0 && (!dC || !B || lddc < N || ldb < N)) return BWS_EINVAL;
-  if (precision == BWS_PREC_FP32) {
-    if (dtype != BWS_DTYPE_F32) return BWS_EUNSUPPORTED;
+  if (precision == BWS_PREC_FP32)
     return sddmm_f32_launch(A, static_cast<const float*>(dC), lddc,
                             static_cast<const float*>(B), ldb, N, dA_values, stream);
-  }
-  return BWS_EUNSUPPORTED;
-}
-
-size_t bws_spmm_batched_workspace_size(int32_t count) {
-  return count < 0 ? 0 : align_up((size_t)count * sizeof(bws_problem), 256) + 256;
-}
-
-bws_status bws_spmm_batched(const bws_problem* problems, int32_t count, int32_t b_dtype,
-                            int32_t precision, const bws_plan* plan, void* workspace,
-                            size_t workspace_bytes, bws_stream stream) {
-  (void)workspace;
-  (void)workspace_bytes;
-  if (count < 0 || (count > 0 && !problems)) return BWS_EINVAL;
-  for (int32_t i = 0; i < count; ++i) {
-    const bws_problem& p = problems[i];
-    BWS_TRY(bws_spmm(&p.A, p.B, b_dtype, p.ldb, p.N, p.C, p.ldc, 1.0f, 0.0f, precision, plan,
-                     nullptr, 0, stream));
-  }
-  return BWS_OK;
+  if (!workspace || workspace_bytes < sddmm_mfma_workspace(A)) return BWS_EWORKSPACE;
+  return sddmm_mfma_launch(A, dC, B, dtype, lddc, ldb, N, dA_values, precision, workspace, stream);
 }
 
 }  // extern "C"

@@ -167,12 +167,13 @@ __device__ __forceinline__ void store_transposed(const uint4 (&rows)[8], uint16_
   }
 }
 
+// One 128 x 128 output tile of one problem (tile index column-block-major).
 template <int PREC, int TB>
-__global__ void __launch_bounds__(256, 2)
-k_spmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-            const void* __restrict__ Bv, int64_t ldb, int64_t N, float* __restrict__ C,
-            int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
-            int64_t per_xcd, int vec) {
+__device__ __forceinline__ void mfma_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K,
+                                          int64_t ldw, const void* __restrict__ Bv, int64_t ldb,
+                                          int64_t N, float* __restrict__ C, int64_t ldc,
+                                          float alpha, float beta, int64_t tile,
+                                          int64_t row_blocks, int vec) {
   constexpr int NS = PREC == 3 ? 2 : 1;
   __shared__ __attribute__((aligned(16))) uint16_t sBt[2][NS][BN][LDK];
   __shared__ __attribute__((aligned(16))) uint32_t sA[2][BM][4];
@@ -180,8 +181,6 @@ k_spmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
-  if (tile >= row_blocks * col_blocks) return;
   const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
   const int64_t m0 = rb * BM, n0 = cb * BN;
   const int wm = wave >> 1, wn = wave & 1;
@@ -306,6 +305,35 @@ k_spmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
 }
 
 template <int PREC, int TB>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+            const void* __restrict__ Bv, int64_t ldb, int64_t N, float* __restrict__ C,
+            int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
+            int64_t per_xcd, int vec) {
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  mfma_tile<PREC, TB>(W, M, K, ldw, Bv, ldb, N, C, ldc, alpha, beta, tile, row_blocks, vec);
+}
+
+// Grouped launch over a problem table (see BatchEntry); vec applies to all.
+template <int PREC, int TB>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma_batch(const BatchEntry* __restrict__ entries, const int64_t* __restrict__ tile_off,
+                  int count, int64_t per_xcd, int vec) {
+  const int64_t g = xcd_tile(blockIdx.x, per_xcd);
+  if (g >= tile_off[count]) return;
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const BatchEntry& e = entries[lo];
+  mfma_tile<PREC, TB>(e.W, e.M, e.K, e.ldw, e.B, e.ldb, e.N, e.C, e.ldc, 1.0f, 0.0f,
+                      g - tile_off[lo], e.row_blocks, vec);
+}
+
+template <int PREC, int TB>
 bws_status launch_mfma(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, float* C,
                        int64_t ldc, float alpha, float beta, hipStream_t s) {
   const int64_t row_blocks = ceil_div(A->rows, BM);
@@ -320,7 +348,35 @@ bws_status launch_mfma(const bws_bitmap* A, const void* B, int64_t ldb, int64_t 
   return launch_check();
 }
 
+template <int PREC, int TB>
+bws_status launch_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
+                             int64_t total_tiles, int vec, hipStream_t s) {
+  const int64_t per_xcd = ceil_div(total_tiles, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  if (total_tiles == 0) return BWS_OK;
+  hipLaunchKernelGGL((k_spmm_mfma_batch<PREC, TB>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0,
+                     s, entries, tile_off, count, per_xcd, vec);
+  return launch_check();
+}
+
 }  // namespace
+
+bws_status spmm_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
+                           int64_t total_tiles, int32_t b_dtype, int32_t precision, int vec,
+                           hipStream_t s) {
+#define BWS_MB(P, T) return launch_mfma_batch<P, T>(entries, tile_off, count, total_tiles, vec, s)
+  if (precision == BWS_PREC_BF16) {
+    if (b_dtype == BWS_DTYPE_BF16) BWS_MB(1, 1);
+    if (b_dtype == BWS_DTYPE_F32) BWS_MB(1, 0);
+  } else if (precision == BWS_PREC_FP16) {
+    if (b_dtype == BWS_DTYPE_F16) BWS_MB(2, 2);
+    if (b_dtype == BWS_DTYPE_F32) BWS_MB(2, 0);
+  } else if (precision == BWS_PREC_BF16X2) {
+    if (b_dtype == BWS_DTYPE_F32) BWS_MB(3, 0);
+  }
+#undef BWS_MB
+  return BWS_EUNSUPPORTED;
+}
 
 bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,

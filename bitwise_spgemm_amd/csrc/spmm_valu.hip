@@ -245,21 +245,21 @@ __device__ __forceinline__ void take_bit(uint64_t& m, int& k) {
   m ^= 1ull << k;
 }
 
+// One output tile ; the
+// tile index is column-block-major within the problem.
 template <int WAVES, int RPW, int KS, bool VALS>
-__global__ void __launch_bounds__(WAVES * 64)
-k_spmm_rows2(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-             const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
-             const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
-             int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
-             int64_t per_xcd) {
+__device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K,
+                                           int64_t ldw, const int64_t* __restrict__ row_ptr,
+                                           const float* __restrict__ values,
+                                           const float* __restrict__ B, int64_t ldb, int64_t N,
+                                           float* __restrict__ C, int64_t ldc, float alpha,
+                                           float beta, int64_t tile, int64_t row_blocks) {
   static_assert(KS == 32 || KS == 64, "stage is one or two bitmap words");
   constexpr int WPS = KS / 32;  // bitmap words per stage
   __shared__ float4 sB0[KS][64];
   __shared__ float4 sB1[KS][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
-  if (tile >= row_blocks * col_blocks) return;
   const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
   const int64_t n0 = cb * kBN;
   const int64_t mw = rb * (WAVES * RPW) + (int64_t)wave * RPW;
@@ -439,6 +439,39 @@ k_spmm_rows2(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   }
 }
 
+template <int WAVES, int RPW, int KS, bool VALS>
+__global__ void __launch_bounds__(WAVES * 64)
+k_spmm_rows2(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+             const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
+             const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
+             int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
+             int64_t per_xcd) {
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  rows2_tile<WAVES, RPW, KS, VALS>(W, M, K, ldw, row_ptr, values, B, ldb, N, C, ldc, alpha, beta,
+                                   tile, row_blocks);
+}
+
+// Grouped launch: workgroup g takes global tile xcd_tile(g), finds its problem
+// by binary search over the tile offsets and runs that problem's tile.
+template <int WAVES, int RPW, int KS, bool VALS>
+__global__ void __launch_bounds__(WAVES * 64)
+k_spmm_rows2_batch(const BatchEntry* __restrict__ entries, const int64_t* __restrict__ tile_off,
+                   int count, int64_t per_xcd) {
+  const int64_t g = xcd_tile(blockIdx.x, per_xcd);
+  if (g >= tile_off[count]) return;
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const BatchEntry& e = entries[lo];
+  rows2_tile<WAVES, RPW, KS, VALS>(e.W, e.M, e.K, e.ldw, e.row_ptr, e.values,
+                                   static_cast<const float*>(e.B), e.ldb, e.N, e.C, e.ldc, 1.0f,
+                                   0.0f, g - tile_off[lo], e.row_blocks);
+}
+
 // Grouped evaluation: one wave per (row, 256-column block), values from the
 // group tables (group_vals of the row's group in this word) or, for a spilled
 // row, from A's own values.
@@ -572,7 +605,45 @@ inline int grid_for(int64_t items, int64_t per_block) {
   return (int)g;
 }
 
+template <int WAVES, int RPW, int KS, bool VALS>
+bws_status launch_rows2_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
+                              int64_t total_tiles, hipStream_t s) {
+  const int64_t per_xcd = ceil_div(total_tiles, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  if (total_tiles == 0) return BWS_OK;
+  hipLaunchKernelGGL((k_spmm_rows2_batch<WAVES, RPW, KS, VALS>), dim3((unsigned)(8 * per_xcd)),
+                     dim3(WAVES * 64), 0, s, entries, tile_off, count, per_xcd);
+  return launch_check();
+}
+
 }  // namespace
+
+// Grouped vector-add launch (FP32 B meeting the direct-staging conditions).
+bws_status spmm_valu_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
+                           int64_t total_tiles, const ValuCfg& cfg, bool vals, hipStream_t s) {
+#define BWS_VB(WV, RP, KS)                                                                    \
+  if (cfg.waves == WV && cfg.rows == WV * RP && cfg.kstage == KS)                             \
+    return vals ? launch_rows2_batch<WV, RP, KS, true>(entries, tile_off, count, total_tiles, s) \
+                : launch_rows2_batch<WV, RP, KS, false>(entries, tile_off, count, total_tiles, s);
+  BWS_VB(8, 32, 64)
+  BWS_VB(16, 16, 64)
+  BWS_VB(16, 16, 32)
+  BWS_VB(8, 16, 64)
+  BWS_VB(8, 16, 32)
+  BWS_VB(8, 8, 32)
+  BWS_VB(16, 8, 32)
+  BWS_VB(16, 8, 64)
+  BWS_VB(16, 4, 32)
+#undef BWS_VB
+  return BWS_EINVAL;
+}
+
+// Whether bws_spmm's FP32 direct-staging kernel applies to these operands.
+bool valu_direct_ok(int32_t b_dtype, const void* B, int64_t ldb, int64_t N, const float* C,
+                    int64_t ldc) {
+  return b_dtype == BWS_DTYPE_F32 && ldb % 4 == 0 && ldc % 4 == 0 && N % 4 == 0 && N >= 4 &&
+         reinterpret_cast<uintptr_t>(B) % 16 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0;
+}
 
 // Vector-add path launcher shared with the dispatcher in spmm.cpp.
 bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,

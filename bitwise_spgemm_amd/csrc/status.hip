@@ -31,6 +31,7 @@ const char* bws_status_string(bws_status status) {
     case BWS_EHIP: return "BWS_EHIP: a HIP call failed";
     case BWS_EWORKSPACE: return "BWS_EWORKSPACE: workspace missing or too small";
     case BWS_ENODEVICE: return "BWS_ENODEVICE: no usable gfx950 device";
+    case BWS_ENOTFOUND: return "BWS_ENOTFOUND: no cached autotuner entry";
     default: return "BWS_UNKNOWN";
   }
 }

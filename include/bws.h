@@ -49,7 +49,8 @@ enum {
   BWS_EUNSUPPORTED = -3, /* valid but unsupported combination (dtype/precision) */
   BWS_EHIP = -4,         /* a HIP call failed; see bws_last_hip_error()         */
   BWS_EWORKSPACE = -5,   /* workspace pointer NULL or smaller than required     */
-  BWS_ENODEVICE = -6     /* no usable gfx950 device                             */
+  BWS_ENODEVICE = -6,    /* no usable gfx950 device                             */
+  BWS_ENOTFOUND = -7     /* no cached autotuner entry for the shape class       */
 };
 
 typedef enum { BWS_DTYPE_F32 = 0, BWS_DTYPE_BF16 = 1, BWS_DTYPE_F16 = 2 } bws_dtype;
@@ -285,13 +286,27 @@ bws_status bws_spmm_tA(const bws_bitmap* AT, const void* dC, int32_t dc_dtype, i
 
 /* SDDMM adjoint on the bitmap pattern: for the j-th set bit (r, k) of row r,
  * dA[row_ptr[r] + j] = Σ_n dC[r, n] · B[k, n].  A.row_ptr is required; A.values
- * is ignored.  dC and B share dtype `dtype` (F32 for FP32, BF16/F16 or F32 for
- * the 16-bit paths, F32 for BF16X2). */
+ * is ignored.  dC and B share dtype `dtype`: F32 for FP32 (vector FMAs) and
+ * BF16X2; BF16/F16 or F32 (rounded on the device) for BF16/FP16, which run on
+ * MFMA over 16 x 16 output blocks with FP32 accumulate and store only the set
+ * bits.  The 16-bit paths need the workspace (per-word popcount offsets). */
+size_t bws_sddmm_workspace_size(const bws_bitmap* A);
 bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
                      int64_t lddc, int64_t ldb, int64_t N, float* dA_values,
-                     int32_t precision, bws_stream stream);
+                     int32_t precision, void* workspace, size_t workspace_bytes,
+                     bws_stream stream);
 
-/* Grouped launch over many bitmap problems in one kernel launch. */
+/* Grouped launch over many bitmap problems: C_i = A_i · B_i (alpha = 1,
+ * beta = 0) for every problem, with one kernel launch per tile configuration.
+ * bws_batch_prepare (host, synchronous: call once per problem set and never
+ * inside graph capture) validates the HOST array `problems`, picks each
+ * problem's tile configuration (the plan's, or the autotuner's cached choice
+ * for the problem's shape class, or the default), and writes the device
+ * problem table and tile offsets into `workspace`.  bws_batch_run enqueues the
+ * launches on `stream` and is capturable into a hipGraph; the workspace must
+ * stay untouched between the two.  Workgroups find their problem by a binary
+ * search over the tile offsets, so problems of very different sizes share a
+ * launch without padding. */
 typedef struct {
   bws_bitmap A;      /* M x K                                     */
   const void* B;     /* K x N, dtype of the launch                */
@@ -299,15 +314,86 @@ typedef struct {
   int64_t N;
   float* C;          /* M x N FP32                                */
   int64_t ldc;
+  int64_t nnz_hint;  /* set bits of A if known (autotuner class), else 0 */
 } bws_problem;
 
+/* Host-side description of a prepared batch (filled by bws_batch_prepare). */
+#define BWS_BATCH_MAX_LAUNCH 16
+typedef struct {
+  int32_t kind;          /* 1 grouped vector-add, 2 grouped MFMA, 3 per-problem launches */
+  int32_t rows_per_wg;   /* tile of kind 1                                              */
+  int32_t waves_per_wg;
+  int32_t k_stage;
+  int32_t values;        /* kind 1: the launch's masks carry values                     */
+  int32_t vec;           /* kind 2: 16-byte B loads allowed for every problem            */
+  int32_t first;         /* first table entry of the launch                              */
+  int32_t count;         /* entries (kinds 1, 2) or problems (kind 3)                    */
+  int64_t total_tiles;   /* workgroup tiles of the launch                                */
+} bws_batch_launch;
+
+typedef struct {
+  int32_t count;
+  int32_t b_dtype;
+  int32_t precision;
+  int32_t n_launch;
+  bws_batch_launch launch[BWS_BATCH_MAX_LAUNCH];
+  void* workspace;                /* device problem table and tile offsets          */
+  const bws_problem* problems;    /* the caller's array; kind 3 launches read it at run
+                                     time, so it must outlive the batch             */
+  bws_plan plan;                  /* the plan given to prepare (kernel 0 = none)    */
+} bws_batch;
+
+size_t bws_batch_workspace_size(int32_t count);
+bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t b_dtype,
+                             int32_t precision, const bws_plan* plan, void* workspace,
+                             size_t workspace_bytes, bws_batch* batch);
+bws_status bws_batch_run(const bws_batch* batch, bws_stream stream);
+
+/* Convenience: prepare + run in one call (not capturable: prepare copies). */
 size_t bws_spmm_batched_workspace_size(int32_t count);
-/* problems is a HOST array (count entries); the library packs it into the
- * workspace with one hipMemcpyAsync, so capture the call after the problem
- * table is final.  C = A·B for every problem (alpha = 1, beta = 0). */
 bws_status bws_spmm_batched(const bws_problem* problems, int32_t count, int32_t b_dtype,
                             int32_t precision, const bws_plan* plan, void* workspace,
                             size_t workspace_bytes, bws_stream stream);
+
+/* Autotuner: times the candidate tile configurations of bws_spmm for this
+ * problem on `stream` (HIP events; the call synchronises and overwrites C)
+ * and returns the fastest in *best, also recording it in the process-wide
+ * cache under the problem's shape class (M, K, N rounded up to powers of two,
+ * log2 of the density rounded to an integer, precision).  bws_batch_prepare
+ * consults the cache for problems with a nonzero nnz_hint when it is given no
+ * plan; bws_spmm never does (its NULL plan means the built-in defaults).  nnz
+ * is the bitmap's set-bit count (the caller knows it).
+ * bws_autotune_save / _load write and read the cache as a text file. */
+bws_status bws_autotune(const bws_bitmap* A, int64_t nnz, const void* B, int32_t b_dtype,
+                        int64_t ldb, int64_t N, float* C, int64_t ldc, int32_t precision,
+                        int32_t repeats, bws_plan* best, float* best_ms, bws_stream stream);
+bws_status bws_autotune_lookup(int64_t M, int64_t K, int64_t N, int64_t nnz, int32_t precision,
+                               bws_plan* plan);
+bws_status bws_autotune_clear(void);
+bws_status bws_autotune_save(const char* path);
+bws_status bws_autotune_load(const char* path);
+
+/* ------------------------------------------------------------------------ */
+/* Sparse-operand phases of the reference pipeline                           */
+/* ------------------------------------------------------------------------ */
+
+/* Symbolic product of two bitmaps: WC (M x N) has bit (r, n) set iff some k
+ * has bit (r, k) in WA and bit (k, n) in WB (the output mask of pre_spgemm,
+ * source/src/transform.cuh:1028-1061).  Output rows are then grouped with
+ * bws_group_rows on WC (the output-row grouping of transform.cuh:1065-1138). */
+bws_status bws_symbolic_or(const uint32_t* WA, int64_t M, int64_t K, int64_t ldwa,
+                           const uint32_t* WB, int64_t N, int64_t ldwb, uint32_t* WC,
+                           int64_t ldwc, bws_stream stream);
+
+/* int8-valued A held as 8 packed bit-planes P (bws_pack_bitplanes_i8):
+ * C = alpha * Σ_b w_b (P_b · B) + beta * C with w = [1,2,4,...,64,-128], the
+ * reference's plane recombination (source/src/main.cu:889-912,
+ * transform.cuh:2245) run as eight 0/1 products.  FP32 rounding is bounded by
+ * Σ_b |w_b| (P_b |B|), not by |A||B|. */
+bws_status bws_spmm_bitplanes_i8(const uint32_t* P, int64_t M, int64_t K, int64_t ldw,
+                                 const void* B, int32_t b_dtype, int64_t ldb, int64_t N, float* C,
+                                 int64_t ldc, float alpha, float beta, int32_t precision,
+                                 const bws_plan* plan, bws_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side deterministic generators (no GPU; xoshiro256** seeded by        */

@@ -22,7 +22,7 @@ def test_exports_every_declared_symbol(bws):
 
 def test_status_strings_and_version(bws):
     L = bws.lib()
-    for code in (0, -1, -2, -3, -4, -5, -6):
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
         assert L.bws_status_string(code).startswith(b"BWS_")
     assert L.bws_status_string(123) == b"BWS_UNKNOWN"
     assert L.bws_version() >> 16 == 1
@@ -121,13 +121,44 @@ def test_spmm_validation(bws):
 def test_sddmm_and_groups_validation(bws):
     L = bws.lib()
     a = _bm(64, 64, 2)  # no row_ptr: SDDMM needs it
-    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 0, 64, 64, 64, FAKE, 0, None) == bws.BWS_EINVAL
+    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 0, 64, 64, 64, FAKE, 0, None, 0, None) == bws.BWS_EINVAL
     a = _bm(64, 64, 2, row_ptr=FAKE)
-    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 0, 63, 64, 64, FAKE, 0, None) == bws.BWS_EINVAL
-    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 1, 64, 64, 64, FAKE, 0, None) == bws.BWS_EUNSUPPORTED
+    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 0, 63, 64, 64, FAKE, 0, None, 0, None) == bws.BWS_EINVAL
+    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 1, 64, 64, 64, FAKE, 0, None, 0, None) == bws.BWS_EUNSUPPORTED
+    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 2, 64, 64, 64, FAKE, 1, None, 0, None) == bws.BWS_EUNSUPPORTED
+    assert L.bws_sddmm(C.byref(a), FAKE, FAKE, 1, 64, 64, 64, FAKE, 1, None, 0, None) == bws.BWS_EWORKSPACE
+    assert L.bws_sddmm_workspace_size(C.byref(a)) >= 64 * 2 * 4
     assert L.bws_spmm_groups(C.byref(a), FAKE, FAKE, 0, 2, FAKE, 64, 64, FAKE, 64, None) == bws.BWS_EINVAL
     assert L.bws_spmm_batched(None, 3, 0, 0, None, None, 0, None) == bws.BWS_EINVAL
-    assert L.bws_spmm_batched(None, 0, 0, 0, None, None, 0, None) == bws.BWS_OK
+    assert L.bws_spmm_batched(None, 0, 0, 0, None, FAKE, L.bws_batch_workspace_size(0), None) == bws.BWS_OK
+    b = bws._Batch()
+    assert L.bws_batch_prepare(None, 0, 0, 0, None, FAKE, 4096, C.byref(b)) == bws.BWS_OK
+    assert b.n_launch == 0 and L.bws_batch_run(C.byref(b), None) == bws.BWS_OK
+    assert L.bws_batch_prepare(None, 0, 0, 0, None, None, 0, C.byref(b)) == bws.BWS_EWORKSPACE
+    assert L.bws_batch_prepare(None, 0, 1, 3, None, FAKE, 4096, C.byref(b)) == bws.BWS_EUNSUPPORTED
+    arr = (bws._Problem * 1)()
+    arr[0] = bws._Problem(_bm(64, 64, 1), FAKE, 64, 64, FAKE, 64, 0)   # ldw too small
+    ws = L.bws_batch_workspace_size(1)
+    assert L.bws_batch_prepare(arr, 1, 0, 0, None, FAKE, ws, C.byref(b)) == bws.BWS_EINVAL
+
+
+def test_autotune_cache_host(bws, tmp_path):
+    L = bws.lib()
+    assert L.bws_autotune_clear() == bws.BWS_OK
+    p = bws._Plan()
+    assert L.bws_autotune_lookup(4096, 4096, 4096, 838861, 0, C.byref(p)) == bws.BWS_ENOTFOUND
+    f = tmp_path / "cache.txt"
+    f.write_text("# comment\n0 12 12 12 -4 1 256 16 64\n1 10 10 10 -3 2 0 0 0\n")
+    assert L.bws_autotune_load(str(f).encode()) == bws.BWS_OK
+    assert L.bws_autotune_lookup(4096, 4096, 4096, 838861, 0, C.byref(p)) == bws.BWS_OK
+    assert (p.kernel, p.rows_per_wg, p.waves_per_wg, p.k_stage) == (1, 256, 16, 64)
+    assert L.bws_autotune_lookup(1000, 1000, 1000, 100000, 1, C.byref(p)) == bws.BWS_OK
+    assert p.kernel == 2
+    g = tmp_path / "out.txt"
+    assert L.bws_autotune_save(str(g).encode()) == bws.BWS_OK
+    assert len([l for l in g.read_text().splitlines() if not l.startswith("#")]) == 2
+    assert L.bws_autotune_load(b"/nonexistent/x") == bws.BWS_EINVAL
+    L.bws_autotune_clear()
 
 
 def test_generators_deterministic(bws):

@@ -1,0 +1,123 @@
+"""GPU parity of the adjoints: SDDMM on the bitmap pattern (FP32 and MFMA
+precisions), the transposed-bitmap SpMM, and the autograd harness checked
+against FP64 central finite differences (on the CPU oracle) and a dense FP64
+reference gradient.
+
+Error scale for an SDDMM value: Σ_n |dC[r,n]|·|B[k,n]| (FP64)."""
+import numpy as np
+import pytest
+import torch
+
+from _helpers import make_problem, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def sddmm_err(oracle, W, K, rp, dC, B, got):
+    R = oracle.sddmm_f64(W, K, rp, dC, B)
+    S = oracle.sddmm_f64(W, K, rp, dC, B, absolute=True)
+    return float(np.max(np.abs(got - R) / np.maximum(S, 1e-300))) if len(R) else 0.0
+
+
+@pytest.mark.parametrize("M,K,N,p", [(128, 128, 64, 0.1), (300, 500, 200, 0.05), (17, 33, 3, 0.5),
+                                     (1024, 512, 1000, 0.02), (64, 64, 32, 1.0)])
+@pytest.mark.parametrize("prec,dt", [("bf16", torch.bfloat16), ("bf16", torch.float32),
+                                     ("fp16", torch.float16), ("fp16", torch.float32),
+                                     ("bf16x2", torch.float32)])
+def test_sddmm_mfma(oracle, bws, dev, M, K, N, p, prec, dt):
+    A, W, rp, _, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M + N)
+    dC = bws.gen_f32(7, (M, N), "normal", 0.0, 1.0)
+    A = bws.Bitmap(A.words, M, K, to_dev(rp, dev))
+    dCd, Bx = to_dev(dC, dev).to(dt), Bd.to(dt)
+    got = bws.sddmm(A, dCd, Bx, prec).cpu().numpy()
+    if prec == "bf16x2":
+        assert sddmm_err(oracle, W, K, rp, dC, B, got) <= 5e-5
+        return
+    rnd = oracle.round_bf16 if prec == "bf16" else oracle.round_f16
+    # mechanics: against the operands the kernel actually multiplies
+    assert sddmm_err(oracle, W, K, rp, rnd(dC), rnd(B), got) <= 2e-5
+    bound = 1e-2 if prec == "bf16" else 2e-3
+    assert sddmm_err(oracle, W, K, rp, dC, B, got) <= bound
+
+
+def test_sddmm_fp32_strided(oracle, bws, dev):
+    M, K, N = 200, 300, 130
+    A, W, rp, _, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=3)
+    dC = bws.gen_f32(8, (M, N + 3), "normal", 0.0, 1.0)
+    A = bws.Bitmap(A.words, M, K, to_dev(rp, dev))
+    got = bws.sddmm(A, to_dev(dC, dev)[:, :N], Bd, "fp32").cpu().numpy()
+    assert sddmm_err(oracle, W, K, rp, np.ascontiguousarray(dC[:, :N]), B, got) <= 1e-5
+
+
+def _dense(oracle, W, K, rp, vals):
+    M = W.shape[0]
+    D = np.zeros((M, K), np.float64)
+    mask = oracle.unpack_bitmap(W, K) != 0
+    D[mask] = vals
+    return D, mask
+
+
+def test_autograd_finite_differences(oracle, bws, dev):
+    """C4's FD check: M = K = 256, N = 128, value-carrying mask, FP64 central
+    differences of L = <G, C> computed on the CPU oracle."""
+    from bitwise_spgemm_amd.autograd import bitmap_spmm
+    M, K, N = 256, 256, 128
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=11, values=True)
+    G = bws.gen_f32(12, (M, N), "normal", 0.0, 1.0)
+    v = A.values.clone().requires_grad_(True)
+    Bt = Bd.clone().requires_grad_(True)
+    C = bitmap_spmm(A, v, Bt, "fp32")
+    (C * to_dev(G, dev)).sum().backward()
+    gv = v.grad.cpu().numpy().astype(np.float64)
+    gB = Bt.grad.cpu().numpy().astype(np.float64)
+
+    def loss(values, Bm):
+        return float((oracle.spmm_f64(W, K, Bm.astype(np.float32), rp, values.astype(np.float32)) *
+                      G).sum())
+
+    rng = np.random.default_rng(0)
+    h = 1e-3
+    vals64 = vals.astype(np.float64)
+    for i in rng.choice(len(vals), 12, replace=False):
+        vp, vm = vals64.copy(), vals64.copy()
+        vp[i] += h
+        vm[i] -= h
+        fd = (loss(vp, B) - loss(vm, B)) / (2 * h)
+        # L is linear in each value: the FD is exact up to the float32 rounding of v ± h
+        assert abs(fd - gv[i]) <= 1e-3 * (abs(fd) + 1e-3), (i, fd, gv[i])
+    for (r, c) in zip(rng.integers(0, K, 12), rng.integers(0, N, 12)):
+        Bp, Bm_ = B.astype(np.float64), B.astype(np.float64)
+        Bp = Bp.copy(); Bm_ = Bm_.copy()
+        Bp[r, c] += h
+        Bm_[r, c] -= h
+        fd = (loss(vals64, Bp) - loss(vals64, Bm_)) / (2 * h)
+        assert abs(fd - gB[r, c]) <= 1e-3 * (abs(fd) + 1e-3), (r, c, fd, gB[r, c])
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16", 1e-2), ("bf16x2", 5e-5)])
+def test_autograd_dense_reference(oracle, bws, dev, prec, tol):
+    """C4-shaped gradients (reduced to 1024 x 1024 x 512) against a dense FP64 reference."""
+    from bitwise_spgemm_amd.autograd import bitmap_spmm
+    M, K, N = 1024, 1024, 512
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=21, values=True)
+    G = bws.gen_f32(22, (M, N), "normal", 0.0, 1.0)
+    v = A.values.clone().requires_grad_(True)
+    Bt = Bd.clone().requires_grad_(True)
+    C = bitmap_spmm(A, v, Bt, prec)
+    (C * to_dev(G, dev)).sum().backward()
+    D, mask = _dense(oracle, W, K, rp, vals)
+    G64, B64 = G.astype(np.float64), B.astype(np.float64)
+    # forward
+    Cref = D @ B64
+    Sref = np.abs(D) @ np.abs(B64)
+    assert np.max(np.abs(C.detach().cpu().numpy() - Cref) / np.maximum(Sref, 1e-300)) <= tol
+    # dB = Aᵀ G
+    gB = Bt.grad.cpu().numpy()
+    R = D.T @ G64
+    S = np.abs(D).T @ np.abs(G64)
+    assert np.max(np.abs(gB - R) / np.maximum(S, 1e-300)) <= tol
+    # dvalues = (G Bᵀ) at the mask
+    gv = v.grad.cpu().numpy()
+    Rv = (G64 @ B64.T)[mask]
+    Sv = (np.abs(G64) @ np.abs(B64).T)[mask]
+    assert np.max(np.abs(gv - Rv) / np.maximum(Sv, 1e-300)) <= tol
