@@ -49,25 +49,34 @@ def time_loop(fn, steps, warmup):
 
 
 def cpu_baseline(W, B, K, N, target_s, threads):
-    """The reference's CPU naive loop (backup/bit_wise_gemm.cu:13-32) on the
-    dense 0/1 form of a bounded row sample; reported in the metric's unit."""
+    """The reference's CPU naive loop (backup/bit_wise_gemm.cu:13-32: i, j, p
+    order, FP32 accumulate, dense over all K, stride-N reads of B) on the dense
+    0/1 form of a bounded row sample of the same A; reported in the metric's
+    unit.  One calibration pass of `threads` rows sizes the timed sample to
+    about `target_s` seconds."""
     import numpy as np
 
     import oracle
-    rows = 8
-    while True:
+    M = W.shape[0]
+
+    def run(rows):
         Ad = oracle.unpack_bitmap(W[:rows], K)
         t0 = time.perf_counter()
         oracle.check_dense_f32(Ad, B, 0, rows, threads=threads)
-        dt = time.perf_counter() - t0
-        if dt >= 0.2 * target_s or rows >= W.shape[0]:
-            break
-        rows = min(W.shape[0], max(rows * 2, int(rows * 0.25 * target_s / max(dt, 1e-3))))
+        return time.perf_counter() - t0
+
+    rows = min(M, threads)
+    dt = run(rows)
+    if dt < 0.5 * target_s and rows < M:
+        rows = int(min(M, max(rows + 1, rows * target_s / max(dt, 1e-6))))
+        rows = min(M, max(threads, rows - rows % threads))
+        dt = run(rows)
     nnz = popcount_u32(np.ascontiguousarray(W[:rows]))
-    return {"value": 2.0 * nnz * N / dt / 1e12, "unit": "TFLOP/s", "cores": threads,
+    return {"value": round(2.0 * nnz * N / dt / 1e12, 6), "unit": "TFLOP/s", "cores": threads,
             "kind": "port",
-            "sample": f"{rows} of {W.shape[0]} rows of the same A (dense 0/1 form) x full B; "
-                      f"check() loop i,j,p FP32, OpenMP over rows; {dt:.2f} s"}
+            "sample": f"first {rows} of {M} rows of the same A (dense 0/1 form, nnz {nnz}) x the "
+                      f"full {K}x{N} B: check() loop i,j,p in FP32, OpenMP over rows, "
+                      f"{dt:.2f} s"}
 
 
 def main():
@@ -85,6 +94,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="extras: skip the C4/C5/comparator legs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -98,6 +110,16 @@ def main():
     import bitwise_spgemm_amd as bws
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings give
+        # the scale between the HBM counters and bytes moved
+        n = args.copy_probe << 18
+        src = torch.ones(n, device=dev)
+        dst = torch.empty(n, device=dev)
+        ms = time_loop(lambda: dst.copy_(src), 5, 1)[1]
+        print(json.dumps({"copy_bytes_each_way": 4 * n, "ms": round(ms, 5),
+                          "GBps_read_plus_write": round(8 * n / ms / 1e6, 1)}), flush=True)
+        return
     M, N, K = args.m, args.n, args.k
     if args.iid:
         W = bws.gen_bitmap_iid(args.seed, M, K, args.p)
@@ -107,8 +129,8 @@ def main():
     nnz = popcount_u32(W)
     A = bws.from_host_words(W, M, K, dev)
     Bd = torch.from_numpy(B).to(dev)
-    Bin = {"fp32": Bd, "bf16x2": Bd, "bf16": Bd.to(torch.bfloat16), "fp16": Bd.to(torch.float16)}[
-        args.precision]
+    Bin = {"fp32": lambda: Bd, "bf16x2": lambda: Bd, "bf16": lambda: Bd.to(torch.bfloat16),
+           "fp16": lambda: Bd.to(torch.float16)}[args.precision]()
     C = torch.empty((M, N), dtype=torch.float32, device=dev)
     log(f"workload M=N=K={M}x{N}x{K} nnz={nnz} density={nnz / (M * K):.4f} prec={args.precision}")
 
@@ -131,7 +153,7 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        extras = run_extras(bws, A, W, Bd, M, N, K, nnz, dev)
+        extras = run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args)
 
     cpu = None
     if not args.no_cpu:
@@ -139,7 +161,7 @@ def main():
         cpu = cpu_baseline(W, B, K, N, args.cpu_seconds, threads)
 
     out = {
-        "metric": "bitmap SpMM TFLOP/s (2*nnz*N/t)",
+        "metric": "bitmap SpMM TFLOP/s (2\u00b7nnz\u00b7N/t) + %roofline; BF16 vs FP32 fwd+bwd at stated err",
         "value": round(value, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -164,33 +186,181 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def run_extras(bws, A, W, Bd, M, N, K, nnz, dev):
-    """Pipeline stages and the BF16 vs FP32 forward+backward comparison."""
+def _err_s(C, R, S):
+    """max / mean of |C - R| / S over all entries (S = |A|·|B|)."""
+    import torch
+    e = (C - R).abs() / S.clamp_min(1e-30)
+    return float(e.max()), float(e.mean())
+
+
+def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args):
+    """Pipeline stages, the precision variants with their error, forward +
+    backward FP32 vs BF16, the value-carrying mask workload, the grouped batch
+    in one captured hipGraph, and library comparators (dense GEMM, CSR SpMM).
+
+    Errors of the 16-bit and split paths are taken against the FP32 path's
+    result, scaled by S = |A|·|B| computed by the same FP32 kernel; the FP32
+    path itself is held within 1e-5 of the FP64 oracle by the parity tests."""
+    import numpy as np
     import torch
     ex = {}
+    tl = lambda f, k=10, w=2: round(time_loop(f, k, w)[1], 4)
     Ad = bws.unpack_bitmap(A)
-    ex["pack_ms"] = round(time_loop(lambda: bws.pack_bitmap(Ad), 5, 2)[1], 4)
-    ex["group_ms"] = round(time_loop(lambda: bws.group_rows(A, 2048, 2), 5, 2)[1], 4)
+    ex["pack_ms"] = tl(lambda: bws.pack_bitmap(Ad), 5, 2)
     del Ad
+    ex["group_ms"] = tl(lambda: bws.group_rows(A, 2048, 2), 5, 2)
+    ex["transpose_ms"] = tl(lambda: bws.transpose(A), 5, 2)
+
+    # precision variants on the same A and B
+    R = bws.spmm(A, Bd, "fp32")
+    S = bws.spmm(A, Bd.abs(), "fp32")
+    C = torch.empty((M, N), device=dev)
+    prec = {}
+    for name, Bx in (("bf16", Bd.to(torch.bfloat16)), ("fp16", Bd.to(torch.float16)),
+                     ("bf16x2", Bd)):
+        ms = tl(lambda: bws.spmm(A, Bx, name, out=C))
+        mx, mean = _err_s(C, R, S)
+        prec[name] = {"fwd_ms": ms, "tflops": round(2.0 * nnz * N / ms / 1e9, 2),
+                      "max_rel_s_vs_fp32": mx, "mean_rel_s_vs_fp32": mean}
+    ex["precision"] = prec
+
+    # forward + backward on the same 0/1 mask (trainable operand B): dB = Aᵀ·dC
     AT = bws.transpose(A)
     dC = torch.randn((M, N), device=dev)
-    Bb = Bd.to(torch.bfloat16)
-    dCb = dC.to(torch.bfloat16)
-    C = torch.empty((M, N), device=dev)
     dB = torch.empty((K, N), device=dev)
 
-    def fb(prec, Bx, dCx):
-        # 0/1 mask: the trainable operand is B, so backward is dB = Aᵀ·dC
-        bws.spmm(A, Bx, prec, out=C)
-        bws.spmm_tA(AT, dCx, prec, out=dB)
-
+    def fb(p, Bx, dCx):
+        bws.spmm(A, Bx, p, out=C)
+        bws.spmm_tA(AT, dCx, p, out=dB)
     try:
-        ex["fwd_bwd_fp32_ms"] = round(time_loop(lambda: fb("fp32", Bd, dC), 3, 1)[1], 4)
-        ex["fwd_bwd_bf16_ms"] = round(time_loop(lambda: fb("bf16", Bb, dCb), 3, 1)[1], 4)
-        ex["fwd_bf16_ms"] = round(time_loop(lambda: bws.spmm(A, Bb, "bf16", out=C), 10, 2)[1], 4)
+        f32 = tl(lambda: fb("fp32", Bd, dC), 5, 2)
+        b16 = tl(lambda: fb("bf16", Bd.to(torch.bfloat16), dC.to(torch.bfloat16)), 5, 2)
+        ex["fwd_bwd_01mask"] = {"fp32_ms": f32, "bf16_ms": b16, "speedup": round(f32 / b16, 3)}
     except Exception as e:  # reported, never hidden
-        ex["fwd_bwd_error"] = str(e)
+        ex["fwd_bwd_01mask"] = {"error": str(e)}
+    del AT, dC, dB, R, S
+
+    if not args.quick:
+        ex["value_mask"] = _value_mask_workload(bws, dev)
+        ex["grouped"] = _grouped_workload(bws, dev)
+        ex["comparators"] = _comparators(bws, A, Bd, M, N, K, nnz, dev)
     return ex
+
+
+def _value_mask_workload(bws, dev):
+    """Value-carrying mask (M = K = 4096, iid 10%, values N(0, 0.02²)) times
+    B 4096x2048: forward, dB = Aᵀ·dC (values transposed each step) and
+    dvalues = (dC·Bᵀ)⊙mask, FP32 and BF16."""
+    import torch
+    M = K = 4096
+    N = 2048
+    W = bws.gen_bitmap_iid(3, M, K, 0.1)
+    A0 = bws.from_host_words(W, M, K, dev)
+    rp = bws.row_ptr(A0)
+    nnz = int(rp[-1].item())
+    vals = torch.from_numpy(bws.gen_f32(4, (nnz,), "normal", 0.0, 0.02)).to(dev)
+    A = A0.with_values(vals, rp)
+    B = torch.from_numpy(bws.gen_f32(5, (K, N), "normal", 0.0, 1.0)).to(dev)
+    dC = torch.randn((M, N), device=dev)
+    C = torch.empty((M, N), device=dev)
+    dB = torch.empty((K, N), device=dev)
+    dv = torch.empty((nnz,), device=dev)
+    AT = bws.transpose(A)
+    out = {"workload": f"iid 0.1 value mask {M}x{K} (nnz {nnz}) @ {K}x{N}"}
+
+    def step(p, Bx, dCx):
+        bws.spmm(A, Bx, p, out=C)
+        ATv = bws.transpose(A)
+        bws.spmm_tA(ATv, dCx, p, out=dB)
+        bws.sddmm(A, dCx, Bx, p, out=dv)
+    try:
+        out["fwd_ms"] = round(time_loop(lambda: bws.spmm(A, B, "fp32", out=C), 10, 2)[1], 4)
+        out["dB_ms"] = round(time_loop(lambda: bws.spmm_tA(AT, dC, "fp32", out=dB), 10, 2)[1], 4)
+        out["dA_fp32_ms"] = round(time_loop(lambda: bws.sddmm(A, dC, B, "fp32", out=dv), 5, 1)[1], 4)
+        Bb, dCb = B.to(torch.bfloat16), dC.to(torch.bfloat16)
+        out["dA_bf16_ms"] = round(time_loop(lambda: bws.sddmm(A, dCb, Bb, "bf16", out=dv), 5, 1)[1], 4)
+        f32 = time_loop(lambda: step("fp32", B, dC), 3, 1)[1]
+        b16 = time_loop(lambda: step("bf16", Bb, dCb), 3, 1)[1]
+        out.update({"fwd_bwd_fp32_ms": round(f32, 4), "fwd_bwd_bf16_ms": round(b16, 4),
+                    "speedup": round(f32 / b16, 3)})
+    except Exception as e:
+        out["error"] = str(e)
+    return out
+
+
+def _grouped_workload(bws, dev):
+    """32 problems, M, K uniform in [256, 8192] (multiples of 64), N = 1024,
+    density uniform in [1%, 20%], alternating iid / clustered masks, BF16 B,
+    forward + backward (dB = Aᵀ·dC) as grouped launches with autotuned tiles,
+    all captured in one hipGraph; a step is one replay."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(0)
+    N = 1024
+    fwd, bwd, flops = [], [], 0.0
+    hints_f, hints_b = [], []
+    tuned = set()
+    for i in range(32):
+        M, K = (int(x) for x in rng.integers(4, 129, 2) * 64)
+        d = float(rng.uniform(0.01, 0.20))
+        W = (bws.gen_bitmap_iid(100 + i, M, K, d) if i % 2 == 0 else
+             bws.gen_bitmap_clustered(100 + i, M, K, 64, d, 0.02))
+        A = bws.from_host_words(W, M, K, dev)
+        nnz = popcount_u32(W)
+        B = torch.randn((K, N), device=dev).to(torch.bfloat16)
+        AT = bws.transpose(A)
+        dC = torch.randn((M, N), device=dev).to(torch.bfloat16)
+        fwd.append((A, B, torch.empty((M, N), device=dev)))
+        bwd.append((AT, dC, torch.empty((K, N), device=dev)))
+        hints_f.append(nnz)
+        hints_b.append(nnz)
+        flops += 2 * 2.0 * nnz * N
+        for (X, Y) in ((A, B), (AT, dC)):
+            key = (X.rows, X.cols)
+            if key not in tuned:
+                bws.autotune(X, Y, "bf16", nnz=nnz, repeats=2)
+                tuned.add(key)
+    fb = bws.Batch(fwd, "bf16", nnz_hints=hints_f)
+    bb = bws.Batch(bwd, "bf16", nnz_hints=hints_b)
+    eager = time_loop(lambda: (fb.run(), bb.run()), 3, 1)[1]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            fb.run()
+            bb.run()
+    torch.cuda.current_stream().wait_stream(s)
+    ms = time_loop(g.replay, 5, 2)[1]
+    return {"problems": 32, "N": N, "fwd_launches": len(fb.launches),
+            "bwd_launches": len(bb.launches), "graph_ms": round(ms, 4),
+            "eager_ms": round(eager, 4), "tflops": round(flops / ms / 1e9, 2)}
+
+
+def _comparators(bws, A, Bd, M, N, K, nnz, dev):
+    """Context numbers on the same operands: dense GEMM (hipBLASLt / rocBLAS
+    through torch.matmul) on the dense 0/1 form of A, and CSR SpMM (rocSPARSE
+    through torch.sparse) — effective TFLOP/s counted as 2·nnz·N like ours."""
+    import torch
+    out = {}
+    Ad = bws.unpack_bitmap(A).to(torch.float32)
+    try:
+        ms = time_loop(lambda: torch.matmul(Ad, Bd), 5, 2)[1]
+        out["dense_fp32_ms"] = round(ms, 4)
+        Ab, Bb = Ad.to(torch.bfloat16), Bd.to(torch.bfloat16)
+        ms = time_loop(lambda: torch.matmul(Ab, Bb), 5, 2)[1]
+        out["dense_bf16_ms"] = round(ms, 4)
+        del Ab, Bb
+    except Exception as e:
+        out["dense_error"] = str(e)
+    try:
+        Acsr = Ad.to_sparse_csr()
+        ms = time_loop(lambda: torch.sparse.mm(Acsr, Bd), 5, 2)[1]
+        out["csr_fp32_ms"] = round(ms, 4)
+        out["csr_fp32_tflops"] = round(2.0 * nnz * N / ms / 1e9, 3)
+    except Exception as e:
+        out["csr_error"] = str(e)
+    return out
 
 
 if __name__ == "__main__":
