@@ -73,7 +73,8 @@ class _Groups(C.Structure):
 
 class _Plan(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("rows_per_wg", C.c_int32), ("waves_per_wg", C.c_int32),
-                ("k_stage", C.c_int32), ("mfma_min_bits", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("k_stage", C.c_int32), ("mfma_min_bits", C.c_int32), ("cols_per_wg", C.c_int32),
+                ("mfma_staging", C.c_int32), ("reserved", C.c_int32)]
 
 
 class _Problem(C.Structure):
@@ -85,8 +86,8 @@ BATCH_MAX_LAUNCH = 16
 
 
 class _BatchLaunch(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("kind", "rows_per_wg", "waves_per_wg", "k_stage",
-                                        "values", "vec", "first", "count")] + \
+    _fields_ = [(n, C.c_int32) for n in ("kind", "rows_per_wg", "cols_per_wg", "waves_per_wg",
+                                        "k_stage", "values", "vec", "first", "count")] + \
                [("total_tiles", C.c_int64)]
 
 
@@ -455,9 +456,13 @@ def group_gather(D: torch.Tensor, A: Bitmap, groups: dict, tile_rows: int, max_g
 # ----------------------------------------------------------------------------
 # (3) grouped tile GEMM
 # ----------------------------------------------------------------------------
+MFMA_STAGING = {"auto": 0, "lut": 1, "regs": 2, "dma3": 3, "dma2": 4}
+
+
 def make_plan(kernel: int = 0, rows_per_wg: int = 0, waves_per_wg: int = 0, k_stage: int = 0,
-              mfma_min_bits: int = 0):
-    return _Plan(kernel, rows_per_wg, waves_per_wg, k_stage, mfma_min_bits, (C.c_int32 * 3)())
+              mfma_min_bits: int = 0, cols_per_wg: int = 0, mfma_staging: str = "auto"):
+    return _Plan(kernel, rows_per_wg, waves_per_wg, k_stage, mfma_min_bits, cols_per_wg,
+                 MFMA_STAGING[mfma_staging], 0)
 
 
 def spmm(A: Bitmap, B: torch.Tensor, precision: str = "fp32", alpha: float = 1.0,
@@ -547,7 +552,8 @@ class Batch:
 
     @property
     def launches(self):
-        return [dict(kind=L.kind, rows=L.rows_per_wg, waves=L.waves_per_wg, k_stage=L.k_stage,
+        return [dict(kind=L.kind, rows=L.rows_per_wg, cols=L.cols_per_wg, waves=L.waves_per_wg,
+                     k_stage=L.k_stage,
                      values=L.values, count=L.count, tiles=L.total_tiles)
                 for L in self.batch.launch[:self.batch.n_launch]]
 

@@ -29,6 +29,14 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
+# Per-file flags.  The SpMM MFMA kernels keep their accumulators in
+# architectural VGPRs (the VGPR form of v_mfma): the wave tiles there need at
+# most 128 accumulator registers, and the AGPR form makes the compiler copy
+# accumulators between register files around the empty-stage branch.
+FILE_FLAGS = {
+    "spmm_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
 
 def _hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
@@ -47,7 +55,7 @@ def sources():
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
-    cmd = [_hipcc()] + HIP_FLAGS + ["-c", src, "-o", obj]
+    cmd = [_hipcc()] + HIP_FLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if src.endswith(".hip"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)

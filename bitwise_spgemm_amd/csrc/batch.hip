@@ -20,8 +20,8 @@ bws_status spmm_dispatch(const bws_bitmap* A, const void* B, int32_t b_dtype, in
 bws_status spmm_valu_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
                            int64_t total_tiles, const ValuCfg& cfg, bool vals, hipStream_t s);
 bws_status spmm_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
-                           int64_t total_tiles, int32_t b_dtype, int32_t precision, int vec,
-                           hipStream_t s);
+                           int64_t total_tiles, const MfmaTile& t, int32_t b_dtype,
+                           int32_t precision, int vec, hipStream_t s);
 bool valu_direct_ok(int32_t b_dtype, const void* B, int64_t ldb, int64_t N, const float* C,
                     int64_t ldc);
 ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N);
@@ -70,11 +70,13 @@ void cache_put(const Key& k, const bws_plan& p) {
 struct Choice {
   int kind = 3;  // 1 grouped vector-add, 2 grouped MFMA, 3 per-problem
   ValuCfg cfg;
+  MfmaTile tile;
   int vals = 0;
   int vec = 0;
   bool operator<(const Choice& o) const {
-    return std::tie(kind, cfg.rows, cfg.waves, cfg.kstage, vals, vec) <
-           std::tie(o.kind, o.cfg.rows, o.cfg.waves, o.cfg.kstage, o.vals, o.vec);
+    return std::tie(kind, cfg.rows, cfg.waves, cfg.kstage, tile.bm, tile.bn, tile.dma, vals, vec) <
+           std::tie(o.kind, o.cfg.rows, o.cfg.waves, o.cfg.kstage, o.tile.bm, o.tile.bn, o.tile.dma,
+                    o.vals, o.vec);
   }
   bool operator==(const Choice& o) const { return !(*this < o) && !(o < *this); }
 };
@@ -93,8 +95,12 @@ Choice classify(const bws_problem& p, int32_t b_dtype, int32_t precision, const 
   if (kernel == 2 && p.A.values) kernel = 1;
   if (kernel == 2) {
     c.kind = 2;
+    c.tile = mfma_tile_for(eff.kernel == 2 ? &eff : nullptr, precision, b_dtype, p.A.rows, p.N);
+    c.tile.legacy = 0;
+    c.tile.nbuf = 2;
     const size_t esz = b_dtype == BWS_DTYPE_F32 ? 4 : 2;
     c.vec = (p.ldb % (int64_t)(16 / esz) == 0) && (reinterpret_cast<uintptr_t>(p.B) % 16 == 0);
+    c.tile.dma = c.tile.dma && dma_ok(precision, b_dtype, p.B, p.ldb, p.N);
     return c;
   }
   if (p.A.cols > 0 && valu_direct_ok(b_dtype, p.B, p.ldb, p.N, p.C, p.ldc)) {
@@ -141,7 +147,10 @@ bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t
   if (!batch || count < 0 || (count > 0 && !problems)) return BWS_EINVAL;
   if (b_dtype < 0 || b_dtype > 2 || precision < 0 || precision > 3) return BWS_EINVAL;
   if (plan && (plan->kernel < 0 || plan->kernel > 2)) return BWS_EINVAL;
-  if (plan && plan->rows_per_wg && !valu_cfg_valid(plan_valu_cfg(plan, 1, 1))) return BWS_EINVAL;
+  if (plan && plan->kernel != 2 && plan->rows_per_wg && !valu_cfg_valid(plan_valu_cfg(plan, 1, 1)))
+    return BWS_EINVAL;
+  if (plan && plan->kernel == 2 && !mfma_tile_valid(plan->rows_per_wg, plan->cols_per_wg))
+    return BWS_EINVAL;
   if (precision == BWS_PREC_FP32 && plan && plan->kernel == 2) return BWS_EUNSUPPORTED;
   if (precision == BWS_PREC_BF16X2 && b_dtype != BWS_DTYPE_F32) return BWS_EUNSUPPORTED;
   if (precision == BWS_PREC_BF16 && b_dtype == BWS_DTYPE_F16) return BWS_EUNSUPPORTED;
@@ -177,11 +186,12 @@ bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t
     if (n_launch >= BWS_BATCH_MAX_LAUNCH) return BWS_ECAPACITY;
     bws_batch_launch& L = batch->launch[n_launch++];
     L.kind = c.kind;
-    L.rows_per_wg = c.cfg.rows;
-    L.waves_per_wg = c.cfg.waves;
-    L.k_stage = c.cfg.kstage;
+    L.rows_per_wg = c.kind == 2 ? c.tile.bm : c.cfg.rows;
+    L.cols_per_wg = c.kind == 2 ? c.tile.bn : 256;
+    L.waves_per_wg = c.kind == 2 ? 4 : c.cfg.waves;
+    L.k_stage = c.kind == 2 ? 16384 / c.tile.bn : c.cfg.kstage;
     L.values = c.vals;
-    L.vec = c.vec;
+    L.vec = c.kind == 2 && c.tile.dma ? 2 : c.vec;
     if (c.kind == 3) {
       L.first = -1;
       L.count = (int32_t)(end - pos);
@@ -203,8 +213,8 @@ bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t
         e.ldb = p.ldb;
         e.N = p.N;
         e.ldc = p.ldc;
-        const int64_t bm = c.kind == 2 ? 128 : c.cfg.rows;
-        const int64_t bn = c.kind == 2 ? 128 : 256;
+        const int64_t bm = c.kind == 2 ? c.tile.bm : c.cfg.rows;
+        const int64_t bn = c.kind == 2 ? c.tile.bn : 256;
         e.row_blocks = ceil_div(e.M, bm);
         e.col_blocks = ceil_div(e.N, bn);
         offs.push_back(tiles);
@@ -249,7 +259,11 @@ bws_status bws_batch_run(const bws_batch* batch, bws_stream stream) {
                               c, L.values != 0, stream));
       off_pos += L.count + 1;
     } else if (L.kind == 2) {
-      BWS_TRY(spmm_mfma_batch(dev_entries + L.first, dev_offs + off_pos, L.count, L.total_tiles,
+      MfmaTile t;
+      t.bm = L.rows_per_wg;
+      t.bn = L.cols_per_wg;
+      t.dma = L.vec == 2;
+      BWS_TRY(spmm_mfma_batch(dev_entries + L.first, dev_offs + off_pos, L.count, L.total_tiles, t,
                               batch->b_dtype, batch->precision, L.vec, stream));
       off_pos += L.count + 1;
     } else {
@@ -300,9 +314,15 @@ bws_status bws_autotune(const bws_bitmap* A, int64_t nnz, const void* B, int32_t
     cand.push_back(p);
   }
   if (precision != BWS_PREC_FP32 && !A->values) {
-    bws_plan p{};
-    p.kernel = 2;
-    cand.push_back(p);
+    static const int32_t tiles[][2] = {{128, 256}, {256, 128}, {128, 128}};
+    for (const auto& t : tiles) {
+      if (precision == BWS_PREC_BF16X2 && t[1] == 256) continue;
+      bws_plan p{};
+      p.kernel = 2;
+      p.rows_per_wg = t[0];
+      p.cols_per_wg = t[1];
+      cand.push_back(p);
+    }
   }
   hipEvent_t e0, e1;
   BWS_TRY(hip_check(hipEventCreate(&e0)));
@@ -355,12 +375,13 @@ bws_status bws_autotune_save(const char* path) {
   FILE* f = fopen(path, "w");
   if (!f) return BWS_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
-  fprintf(f, "# bws autotune cache: precision log2M log2K log2N density_class kernel rows waves kstage\n");
+  fprintf(f, "# bws autotune cache: precision log2M log2K log2N density_class kernel rows waves kstage cols\n");
   for (const auto& kv : cache()) {
     const Key& k = kv.first;
     const bws_plan& p = kv.second;
-    fprintf(f, "%d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
-            std::get<3>(k), std::get<4>(k), p.kernel, p.rows_per_wg, p.waves_per_wg, p.k_stage);
+    fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+            std::get<3>(k), std::get<4>(k), p.kernel, p.rows_per_wg, p.waves_per_wg, p.k_stage,
+            p.cols_per_wg);
   }
   fclose(f);
   return BWS_OK;
@@ -374,9 +395,10 @@ bws_status bws_autotune_load(const char* path) {
   bws_status st = BWS_OK;
   while (fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
-    int v[9];
-    if (sscanf(line, "%d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
-               &v[7], &v[8]) != 9) {
+    int v[10] = {0};
+    const int n = sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4],
+                         &v[5], &v[6], &v[7], &v[8], &v[9]);
+    if (n != 9 && n != 10) {
       st = BWS_EINVAL;
       continue;
     }
@@ -385,6 +407,7 @@ bws_status bws_autotune_load(const char* path) {
     p.rows_per_wg = v[6];
     p.waves_per_wg = v[7];
     p.k_stage = v[8];
+    p.cols_per_wg = v[9];
     cache_put(Key(v[0], v[1], v[2], v[3], v[4]), p);
   }
   fclose(f);

@@ -53,6 +53,21 @@ struct ValuCfg {
 };
 bool valu_cfg_valid(const ValuCfg& c);
 
+// MFMA workgroup tile: bm x bn of C (128 or 256 each); legacy selects the
+// 64 x 64-per-wave kernel with table-lookup A fragments.
+struct MfmaTile {
+  int32_t bm = 128;
+  int32_t bn = 128;
+  int32_t legacy = 0;
+  int32_t dma = 0;   // direct-to-LDS staging of 16-bit B
+  int32_t nbuf = 2;  // LDS stages of the direct-to-LDS kernel
+  int32_t ks = 64;   // K rows per stage of the direct-to-LDS kernel (64 or 128)
+};
+bool dma_ok(int32_t precision, int32_t b_dtype, const void* B, int64_t ldb, int64_t N);
+MfmaTile mfma_tile_for(const bws_plan* plan, int32_t precision, int32_t b_dtype, int64_t M,
+                       int64_t N);
+bool mfma_tile_valid(int bm, int bn);
+
 // One problem of a grouped launch, as the kernels read it from the workspace.
 struct BatchEntry {
   const uint32_t* W;

@@ -47,7 +47,7 @@ ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N) {
 }
 bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,
-                     hipStream_t s);
+                     const bws_plan* plan, hipStream_t s);
 bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
                               int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, hipStream_t s);
@@ -111,7 +111,7 @@ bws_status spmm_dispatch(const bws_bitmap* A, const void* B, int32_t b_dtype, in
   }
   if (kernel == 2) {
     if (A->values) return BWS_EUNSUPPORTED;
-    return spmm_mfma(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, s);
+    return spmm_mfma(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, plan, s);
   }
   return spmm_valu(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, plan_valu_cfg(plan, M, N), s);
 }
@@ -141,8 +141,9 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   if (N < 0 || !dtype_ok(b_dtype) || !prec_ok(precision)) return BWS_EINVAL;
   if (!combo_ok(precision, b_dtype)) return BWS_EUNSUPPORTED;
   if (plan && (plan->kernel < 0 || plan->kernel > 2 || plan->rows_per_wg < 0 ||
-               (plan->rows_per_wg != 0 &&
-                !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N)))))
+               (plan->kernel != 2 && plan->rows_per_wg != 0 &&
+                !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N))) ||
+               (plan->kernel == 2 && !mfma_tile_valid(plan->rows_per_wg, plan->cols_per_wg))))
     return BWS_EINVAL;
   if (A->rows == 0 || N == 0) return BWS_OK;
   if (!C || ldc < N) return BWS_EINVAL;

@@ -1,25 +1,30 @@
 // spmm_mfma.hip — 16-bit MFMA tile GEMM for a 0/1 bitmap A (gfx950,
-// v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulate).
+// v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulate).  Three kernels share the
+// operand layout below:
+//  - k_spmm_mfma4 (default for 16-bit B): workgroup 128 x 128, 256 x 128 or
+//    128 x 256 of C, B copied straight into LDS and read with the transposing
+//    LDS read; see the comment above mfma4_tile.
+//  - k_spmm_mfma3: the same tiles with B loaded into registers, transposed with
+//    v_perm_b32 and written to LDS (FP32 B rounded on the fly, BF16X2, and B
+//    layouts the direct copy cannot take).
+//  - k_spmm_mfma (BWS_MFMA_LUT): 64 x 64 per wave, A fragments from an LDS
+//    table, kept as a reference point for the other two.
+// Operand layout used for both operands: lane l covers row/column (l % 16)
+// and the K slots 8*(l/16) .. 8*(l/16)+7 of the 32-wide step; since A and B use
+// the same K assignment the product does not depend on the hardware's K order,
+// and all kernels accumulate the same K steps in the same order (their results
+// are bit-identical).  The accumulator layout is D[4*(l/16) + e][l % 16].
+// A stage whose bitmap block (a wave's rows x the stage's K columns) is empty
+// skips its MFMAs.
 //
-// Workgroup = 4 waves (2 x 2), C tile 128 x 128, each wave 64 x 64 (4 x 4
-// MFMA tiles).  K advances 128 rows per LDS stage = four 32-wide MFMA steps =
-// four bitmap words, so one barrier covers 64 MFMAs per wave.
+// Below, the table-lookup kernel: workgroup = 4 waves (2 x 2), C tile
+// 128 x 128, each wave 64 x 64 (4 x 4 MFMA tiles), K stage 128 rows.
 //  - A operand: the byte of the bitmap word that a lane's 8 K-slots cover
-//    indexes a 256-entry LDS table of eight 16-bit 0/1 values (one
-//    ds_read_b128); no arithmetic builds the fragment.  A stage whose 64 x 128
-//    A block (one wave's rows) has no set bit skips its MFMAs; otherwise the
-//    MFMAs run branch-free with the fragments of K-step s+1 read while step s
-//    multiplies.
+//    indexes a 256-entry LDS table of eight 16-bit 0/1 values.
 //  - B operand: staged in LDS transposed (N-major, K contiguous, 272-byte rows)
 //    so a lane's 8 K-consecutive values are one 16-byte read.  Each thread
 //    loads an 8(K) x 8(N) block with 16-byte global loads, transposes it in
-//    registers with v_perm_b32 and writes eight 16-byte LDS rows.  The next
-//    stage's global loads are in flight while the current stage computes.
-//  - A words (4 per row per stage) go through LDS as well.
-// Operand layout used for both operands: lane l covers row/column (l % 16)
-// and the K slots 8*(l/16) .. 8*(l/16)+7 of the 32-wide step; since A and B use
-// the same K assignment the product does not depend on the hardware's K order.
-// The accumulator layout is D[4*(l/16) + e][l % 16], e = 0..3.
+//    registers with v_perm_b32 and writes eight 16-byte LDS rows.
 //
 // BF16X2: B = hi + lo with hi = bf16(b), lo = bf16(b - hi); both are staged
 // and both products accumulate into the same FP32 accumulators.
@@ -304,6 +309,452 @@ __device__ __forceinline__ void mfma_tile(const uint32_t* __restrict__ W, int64_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Large-tile variant: 2 x 2 waves, each wave WM x WN of C (64 x 64, 64 x 128 or
+// 128 x 64: at most 128 accumulator VGPRs, so two workgroups fit on a CU), so
+// each LDS byte read feeds up to 2x more MFMAs than a 64 x 64 wave tile.  K stage KS = 16384 / BN, so the B stage is always 256 blocks of
+// 8 (K) x 8 (N), one per thread.  The 0/1 A fragment is built in VALU from the
+// bitmap byte (no table lookup): with the word bit-reversed at staging, K slot
+// 8q + e of the 32-wide step is bit e of byte q; for z = x | (x << 15),
+// (z & (0x10001 << 2p)) * (ONE >> 2p) puts ONE in the low half iff bit 2p is
+// set and in the high half iff bit 2p + 1 is set (ONE, the 16-bit 1.0, has at
+// least 7 trailing zero bits, so ONE >> 2p is exact for p < 4).
+// LDS layout: B transposed [n][KS + 8] (row stride 9 or 17 sixteen-byte units,
+// odd, so the 16 lanes of a fragment read and the 16 lanes of a transposed
+// write each touch 16 distinct bank groups); A words [BM][KW].
+// ---------------------------------------------------------------------------
+template <int PREC, int WM, int WN>
+struct Cfg3 {
+  static constexpr int BM = 2 * WM, BN = 2 * WN;
+  static constexpr int KS = 16384 / BN;  // 64 (BN 256) or 128 (BN 128)
+  static constexpr int KW = KS / 32;
+  static constexpr int LDK = KS + 8;
+  static constexpr int NS = PREC == 3 ? 2 : 1;
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static constexpr int AWPT = BM * KW / 256;  // A words staged per thread
+  static constexpr int KBN = KS / 8;          // 8-row K blocks per stage
+};
+
+template <int PREC>
+__device__ __forceinline__ uint4 a_frag_bits(uint32_t rw, int q) {
+  constexpr uint32_t ONE = PREC == 2 ? 0x3C00u : 0x3F80u;
+  const uint32_t x = __builtin_amdgcn_ubfe(rw, 8 * q, 8);
+  const uint32_t z = x | (x << 15);
+  return make_uint4((uint32_t)__umul24(z & 0x00010001u, ONE),
+                    (uint32_t)__umul24(z & 0x00040004u, ONE >> 2),
+                    (uint32_t)__umul24(z & 0x00100010u, ONE >> 4),
+                    (uint32_t)__umul24(z & 0x00400040u, ONE >> 6));
+}
+
+template <int PREC, int TB, int WM, int WN>
+__device__ __forceinline__ void mfma3_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K,
+                                           int64_t ldw, const void* __restrict__ Bv, int64_t ldb,
+                                           int64_t N, float* __restrict__ C, int64_t ldc,
+                                           float alpha, float beta, int64_t tile,
+                                           int64_t row_blocks, int vec) {
+  using G = Cfg3<PREC, WM, WN>;
+  constexpr int BM = G::BM, BN = G::BN, KS = G::KS, KW = G::KW, LDK = G::LDK, NS = G::NS;
+  constexpr int TM = G::TM, TN = G::TN, AWPT = G::AWPT, KBN = G::KBN;
+  __shared__ __attribute__((aligned(16))) uint16_t sBt[2][NS][BN][LDK];
+  __shared__ __attribute__((aligned(16))) uint32_t sA[2][BM][KW];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
+  const int64_t m0 = rb * BM, n0 = cb * BN;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int j = lane & 15, q = lane >> 4;
+
+  const int64_t nst = (K + KS - 1) / KS;
+  const int64_t wpr = (K + 31) / 32;
+  // staging roles: B block (kb, nb) = K rows 8kb..8kb+7 x N columns 8nb..8nb+7
+  // (kb fastest: conflict-free transposed writes); A words: AWPT consecutive
+  // words of row arow starting at word aoff of the stage
+  const int kb = tid % KBN, nb = tid / KBN;
+  constexpr int TPR = KW / AWPT;  // threads per A row
+  const int arow = tid / TPR, aoff = (tid % TPR) * AWPT;
+  const bool arow_ok = m0 + arow < M;
+
+  BBlock<TB> blk;
+  uint32_t aw[AWPT];
+  auto load_stage = [&](int64_t st) {
+    load_block<TB>(blk, Bv, st * KS + 8 * kb, n0 + 8 * nb, ldb, K, N, vec != 0);
+#pragma unroll
+    for (int x = 0; x < AWPT; ++x) {
+      const int64_t w = KW * st + aoff + x;
+      aw[x] = (arow_ok && w < wpr) ? (W[(m0 + arow) * ldw + w] & valid_bits(w, K)) : 0u;
+    }
+  };
+  auto store_stage = [&](int buf) {
+    uint4 hi[8], lo[8];
+    pack_block<PREC, TB>(blk, hi, lo);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      const uint4 col = make_uint4(pick2(hi[0], hi[1], x), pick2(hi[2], hi[3], x),
+                                   pick2(hi[4], hi[5], x), pick2(hi[6], hi[7], x));
+      *reinterpret_cast<uint4*>(&sBt[buf][0][8 * nb + x][8 * kb]) = col;
+      if constexpr (NS == 2) {
+        const uint4 cl = make_uint4(pick2(lo[0], lo[1], x), pick2(lo[2], lo[3], x),
+                                    pick2(lo[4], lo[5], x), pick2(lo[6], lo[7], x));
+        *reinterpret_cast<uint4*>(&sBt[buf][1][8 * nb + x][8 * kb]) = cl;
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < AWPT; ++x) sA[buf][arow][aoff + x] = __brev(aw[x]);
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  load_stage(0);
+  store_stage(0);
+  __syncthreads();
+
+  for (int64_t st = 0; st < nst; ++st) {
+    const int buf = (int)(st & 1);
+    const bool more = st + 1 < nst;
+    if (more) load_stage(st + 1);
+
+    uint32_t wr[TM][KW];
+    uint32_t any = 0;
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+      const uint32_t* src = &sA[buf][WM * wm + 16 * a + j][0];
+      if constexpr (KW == 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(src);
+        wr[a][0] = v.x, wr[a][1] = v.y, wr[a][2] = v.z, wr[a][3] = v.w;
+      } else {
+        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        wr[a][0] = v.x, wr[a][1] = v.y;
+      }
+#pragma unroll
+      for (int s = 0; s < KW; ++s) any |= wr[a][s];
+    }
+    if (__ballot(any != 0u) != 0ull) {  // skip a stage whose WM x KS A block is empty
+      // B fragments of K-step s+1 are read while step s multiplies; the A
+      // fragment of row tile a+1 is built while row tile a multiplies.
+      uint4 bh[2][TN], bl[2][TN];
+      auto frags = [&](int s, int d) {
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          bh[d][b] = *reinterpret_cast<const uint4*>(
+              &sBt[buf][0][WN * wn + 16 * b + j][32 * s + 8 * q]);
+          if constexpr (NS == 2)
+            bl[d][b] = *reinterpret_cast<const uint4*>(
+                &sBt[buf][1][WN * wn + 16 * b + j][32 * s + 8 * q]);
+        }
+      };
+      frags(0, 0);
+#pragma unroll
+      for (int s = 0; s < KW; ++s) {
+        const int d = s & 1;
+        if (s + 1 < KW) frags(s + 1, d ^ 1);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) {
+          const uint4 af = a_frag_bits<PREC>(wr[a][s], q);
+#pragma unroll
+          for (int b = 0; b < TN; ++b) {
+            acc[a][b] = mfma<PREC>(af, bh[d][b], acc[a][b]);
+            if constexpr (NS == 2) acc[a][b] = mfma<PREC>(af, bl[d][b], acc[a][b]);
+          }
+        }
+      }
+    }
+    if (more) store_stage(buf ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const int64_t col = n0 + WN * wn + 16 * b + j;
+      if (col >= N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = m0 + WM * wm + 16 * a + 4 * q + e;
+        if (row < M) {
+          float* c = C + row * ldc + col;
+          const float v = alpha * acc[a][b][e];
+          *c = beta != 0.0f ? fmaf(beta, *c, v) : v;
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Direct-to-LDS variant for 16-bit B (N and ldb multiples of 8, 16-byte
+// aligned): no register staging at all.
+//  - B stages of 64 K rows are copied row-major into LDS by
+//    global_load_lds_dwordx4 (each wave instruction writes 1 KiB), NBUF stages
+//    in flight.  Within a row, 32-byte unit c is stored at slot c ^ f(k) with
+//    f(k) = (k & 3) | ((k >> 1) & 4), so the transposing reads below hit 8
+//    different slots (all 64 banks) per 32 lanes.
+//  - B fragments come from ds_read_b64_tr_b16: in each group of 16 lanes, lane
+//    i addresses row kb + i/4, columns n + 4(i%4) .. +3, and receives column
+//    n + i of rows kb .. kb+3; two reads give the 8 K values of an MFMA operand
+//    (K slots 8q .. 8q+7 for lane group q, the same slots the A operand uses).
+//  - bitmap words are copied by global_load_lds_dword (4 bytes per lane);
+//    masking of bits beyond K and the bit reversal happen when they are read.
+// Rows beyond K load row K-1 and columns beyond N load columns N-8..N-1: that
+// data meets zero A bits or is never stored.  B must be finite (0 x Inf = NaN
+// in a dense contraction, as in any GEMM).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dma16(const void* g, uint32_t lds_addr) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(g), "s"(lds_addr)
+               : "memory", "m0");
+}
+__device__ __forceinline__ void dma4(const void* g, uint32_t lds_addr) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off"
+               :
+               : "v"(g), "s"(lds_addr)
+               : "memory", "m0");
+}
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+__device__ __forceinline__ uint32_t lds_u32(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+typedef short v4s __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 tr_read(uint32_t lds_addr) {
+  const v4s v = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+      (v4s __attribute__((address_space(3)))*)(uintptr_t)lds_addr);
+  return __builtin_bit_cast(uint2, v);
+}
+
+template <int WM, int WN, int NBUF, int KS_>
+struct Cfg4 {
+  static constexpr int BM = 2 * WM, BN = 2 * WN, KS = KS_, KW = KS_ / 32;
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static constexpr int ROWB = BN * 2;                    // bytes per B row in LDS
+  static constexpr int SB = KS * ROWB;                   // bytes per B stage
+  static constexpr int NB_W = SB / 1024 / 4;             // B DMA instructions per wave per stage
+  static constexpr int NA_W = BM * KW / 64 / 4;          // word DMA instructions per wave
+  static constexpr int PER_STAGE = NB_W + NA_W;
+  static constexpr int LANES_PER_ROW = ROWB / 16;        // 32 (BN 256) or 16 (BN 128)
+};
+
+template <int PREC, int WM, int WN, int NBUF, int KS_>
+__device__ __forceinline__ void mfma4_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K,
+                                           int64_t ldw, const uint16_t* __restrict__ B,
+                                           int64_t ldb, int64_t N, float* __restrict__ C,
+                                           int64_t ldc, float alpha, float beta, int64_t tile,
+                                           int64_t row_blocks) {
+  using G = Cfg4<WM, WN, NBUF, KS_>;
+  constexpr int BM = G::BM, BN = G::BN, KS = G::KS, KW = G::KW, TM = G::TM, TN = G::TN;
+  constexpr int ROWB = G::ROWB, SB = G::SB, NB_W = G::NB_W, NA_W = G::NA_W;
+  constexpr int LPR = G::LANES_PER_ROW;
+  static_assert(NB_W >= 1 && NA_W >= 1, "stage must give every wave DMA work");
+  __shared__ __attribute__((aligned(1024))) uint8_t sB[NBUF][SB];
+  __shared__ __attribute__((aligned(256))) uint32_t sA[NBUF][BM * KW];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
+  const int64_t m0 = rb * BM, n0 = cb * BN;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int j = lane & 15, q = lane >> 4;
+  const int64_t nst = (K + KS - 1) / KS;
+  const int64_t wpr = (K + 31) / 32;
+
+  // per-lane DMA sources (the stage only moves the row index)
+  const uint16_t* bsrc[NB_W];
+  int brow[NB_W];
+#pragma unroll
+  for (int t = 0; t < NB_W; ++t) {
+    const int g = wave * NB_W + t;                 // instruction index within the stage
+    const int r = g * (1024 / ROWB) + lane / LPR;  // B row within the stage
+    const int u = lane % LPR;                      // 16-byte piece of the LDS row
+    const int slot = u >> 1, half = u & 1;
+    const int c = slot ^ ((r & 3) | ((r >> 1) & 4));
+    int64_t col = n0 + 16 * c + 8 * half;
+    if (col + 8 > N) col = N - 8;
+    bsrc[t] = B + col;
+    brow[t] = r;
+  }
+  const uint32_t* asrc[NA_W];
+  int aw[NA_W];
+#pragma unroll
+  for (int t = 0; t < NA_W; ++t) {
+    const int idx = (wave * NA_W + t) * 64 + lane;
+    int64_t row = m0 + idx / KW;
+    if (row > M - 1) row = M - 1;
+    asrc[t] = W + row * ldw;
+    aw[t] = idx % KW;
+  }
+  const uint32_t sB0 = lds_u32(&sB[0][0]), sA0 = lds_u32(&sA[0][0]);
+
+  auto issue = [&](int64_t st) {
+    const int buf = (int)(st % NBUF);
+#pragma unroll
+    for (int t = 0; t < NB_W; ++t) {
+      int64_t k = st * KS + brow[t];
+      if (k > K - 1) k = K - 1;
+      dma16(bsrc[t] + k * ldb, sB0 + (uint32_t)(buf * SB + (wave * NB_W + t) * 1024));
+    }
+#pragma unroll
+    for (int t = 0; t < NA_W; ++t) {
+      int64_t w = st * KW + aw[t];
+      if (w > wpr - 1) w = wpr - 1;
+      dma4(asrc[t] + w, sA0 + (uint32_t)(buf * BM * KW * 4 + (wave * NA_W + t) * 256));
+    }
+  };
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int p = 0; p < NBUF - 1; ++p)
+    if (p < nst) issue(p);
+
+  // lane constants of the transposing reads: row 8q + i/4 (+ 4h + 32 ks),
+  // 8-byte piece i%4 of the slot
+  const int i4 = j >> 2;
+  const int fsw = i4 | ((q & 1) << 2);
+  const uint32_t rd_lane = (uint32_t)((8 * q + i4) * ROWB + 8 * (j & 3));
+  const int c0 = (WN * wn) / 16;
+
+  for (int64_t st = 0; st < nst; ++st) {
+    const int buf = (int)(st % NBUF);
+    if (st + NBUF - 2 < nst) wait_vmcnt<(NBUF - 2) * G::PER_STAGE>();
+    else wait_vmcnt<0>();
+    __syncthreads();
+    if (st + NBUF - 1 < nst) issue(st + NBUF - 1);
+
+    uint32_t wr[TM][KW];
+    uint32_t any = 0;
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+      const uint32_t* src = &sA[buf][(WM * wm + 16 * a + j) * KW];
+      uint32_t v[KW];
+      if constexpr (KW == 4) {
+        const uint4 x = *reinterpret_cast<const uint4*>(src);
+        v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w;
+      } else {
+        const uint2 x = *reinterpret_cast<const uint2*>(src);
+        v[0] = x.x, v[1] = x.y;
+      }
+#pragma unroll
+      for (int x = 0; x < KW; ++x) {
+        wr[a][x] = __brev(v[x] & valid_bits(st * KW + x, K));
+        any |= wr[a][x];
+      }
+    }
+    if (__ballot(any != 0u) != 0ull) {
+      const uint32_t sbase = sB0 + (uint32_t)(buf * SB) + rd_lane;
+      uint4 bf[2][TN];
+      auto frags = [&](int ks, int d) {
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          const uint32_t a0 = sbase + (uint32_t)(32 * ((c0 + b) ^ fsw)) + (uint32_t)(32 * ks * ROWB);
+          const uint2 lo = tr_read(a0);
+          const uint2 hi = tr_read(a0 + 4 * ROWB);
+          bf[d][b] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+      };
+      frags(0, 0);
+#pragma unroll
+      for (int ks = 0; ks < KW; ++ks) {
+        const int d = ks & 1;
+        if (ks + 1 < KW) frags(ks + 1, d ^ 1);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) {
+          const uint4 af = a_frag_bits<PREC>(wr[a][ks], q);
+#pragma unroll
+          for (int b = 0; b < TN; ++b) acc[a][b] = mfma<PREC>(af, bf[d][b], acc[a][b]);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const int64_t col = n0 + WN * wn + 16 * b + j;
+      if (col >= N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = m0 + WM * wm + 16 * a + 4 * q + e;
+        if (row < M) {
+          float* cp = C + row * ldc + col;
+          const float v = alpha * acc[a][b][e];
+          *cp = beta != 0.0f ? fmaf(beta, *cp, v) : v;
+        }
+      }
+    }
+}
+
+template <int PREC, int WM, int WN, int NBUF, int KS>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma4(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+             const uint16_t* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
+             int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
+             int64_t per_xcd) {
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  mfma4_tile<PREC, WM, WN, NBUF, KS>(W, M, K, ldw, B, ldb, N, C, ldc, alpha, beta, tile,
+                                     row_blocks);
+}
+
+template <int PREC, int WM, int WN, int NBUF, int KS>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma4_batch(const BatchEntry* __restrict__ entries, const int64_t* __restrict__ tile_off,
+                   int count, int64_t per_xcd) {
+  const int64_t g = xcd_tile(blockIdx.x, per_xcd);
+  if (g >= tile_off[count]) return;
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const BatchEntry& e = entries[lo];
+  mfma4_tile<PREC, WM, WN, NBUF, KS>(e.W, e.M, e.K, e.ldw, static_cast<const uint16_t*>(e.B),
+                                     e.ldb, e.N, e.C, e.ldc, 1.0f, 0.0f, g - tile_off[lo],
+                                     e.row_blocks);
+}
+
+template <int PREC, int TB, int WM, int WN>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma3(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+             const void* __restrict__ Bv, int64_t ldb, int64_t N, float* __restrict__ C,
+             int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
+             int64_t per_xcd, int vec) {
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  mfma3_tile<PREC, TB, WM, WN>(W, M, K, ldw, Bv, ldb, N, C, ldc, alpha, beta, tile, row_blocks,
+                               vec);
+}
+
+template <int PREC, int TB, int WM, int WN>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma3_batch(const BatchEntry* __restrict__ entries, const int64_t* __restrict__ tile_off,
+                   int count, int64_t per_xcd, int vec) {
+  const int64_t g = xcd_tile(blockIdx.x, per_xcd);
+  if (g >= tile_off[count]) return;
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const BatchEntry& e = entries[lo];
+  mfma3_tile<PREC, TB, WM, WN>(e.W, e.M, e.K, e.ldw, e.B, e.ldb, e.N, e.C, e.ldc, 1.0f, 0.0f,
+                               g - tile_off[lo], e.row_blocks, vec);
+}
+
 template <int PREC, int TB>
 __global__ void __launch_bounds__(256, 2)
 k_spmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
@@ -348,23 +799,177 @@ bws_status launch_mfma(const bws_bitmap* A, const void* B, int64_t ldb, int64_t 
   return launch_check();
 }
 
+template <int PREC, int TB, int WM, int WN>
+bws_status launch_mfma3(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, float* C,
+                        int64_t ldc, float alpha, float beta, hipStream_t s) {
+  const int64_t row_blocks = ceil_div(A->rows, 2 * WM);
+  const int64_t col_blocks = ceil_div(N, 2 * WN);
+  const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  const size_t esz = TB == 0 ? 4 : 2;
+  const int vec = (ldb % (16 / esz) == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+  hipLaunchKernelGGL((k_spmm_mfma3<PREC, TB, WM, WN>), dim3((unsigned)(8 * per_xcd)), dim3(256),
+                     0, s, A->words, A->rows, A->cols, A->ldw, B, ldb, N, C, ldc, alpha, beta,
+                     row_blocks, col_blocks, per_xcd, vec);
+  return launch_check();
+}
+
+// Tiles built per (precision, B type); mfma_tile_for() only returns these.
+//   16-bit B:        128 x 128, 128 x 256, 256 x 128
+//   FP32 B (BF16 / FP16): 128 x 128 (twice the staging registers)
+//   BF16X2:          128 x 128, 256 x 128 (two B planes in LDS)
+template <int PREC, int TB>
+constexpr bool has_tile(int bm, int bn) {
+  if (bm == 128 && bn == 128) return true;
+  if (PREC == 3) return bm == 256 && bn == 128;
+  if (TB == 0) return false;
+  return (bm == 128 && bn == 256) || (bm == 256 && bn == 128);
+}
+
+template <int PREC, int WM, int WN, int NBUF, int KS>
+bws_status launch_mfma4(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, float* C,
+                        int64_t ldc, float alpha, float beta, hipStream_t s) {
+  const int64_t row_blocks = ceil_div(A->rows, 2 * WM);
+  const int64_t col_blocks = ceil_div(N, 2 * WN);
+  const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  hipLaunchKernelGGL((k_spmm_mfma4<PREC, WM, WN, NBUF, KS>), dim3((unsigned)(8 * per_xcd)), dim3(256),
+                     0, s, A->words, A->rows, A->cols, A->ldw, static_cast<const uint16_t*>(B),
+                     ldb, N, C, ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
+  return launch_check();
+}
+
+template <int PREC, int TB>
+bws_status launch_mfma_cfg(const MfmaTile& t, const bws_bitmap* A, const void* B, int64_t ldb,
+                           int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                           hipStream_t s) {
+  if (t.legacy) return launch_mfma<PREC, TB>(A, B, ldb, N, C, ldc, alpha, beta, s);
+  if (!has_tile<PREC, TB>(t.bm, t.bn)) return BWS_EUNSUPPORTED;
+  if constexpr (TB != 0 && PREC != 3) {
+    if (t.dma) {
+#define BWS_L4(WM, WN, NB, KS) return launch_mfma4<PREC, WM, WN, NB, KS>(A, B, ldb, N, C, ldc, alpha, beta, s)
+      if (t.nbuf == 3) {
+        if (t.bn == 256) BWS_L4(64, 128, 3, 64);
+        if (t.bm == 256) BWS_L4(128, 64, 3, 64);
+        BWS_L4(64, 64, 3, 64);
+      }
+      if (t.ks == 128) {
+        if (t.bn == 256) BWS_L4(64, 128, 2, 128);
+        if (t.bm == 256) BWS_L4(128, 64, 2, 128);
+        BWS_L4(64, 64, 2, 128);
+      }
+      if (t.bn == 256) BWS_L4(64, 128, 2, 64);
+      if (t.bm == 256) BWS_L4(128, 64, 2, 64);
+      BWS_L4(64, 64, 2, 64);
+#undef BWS_L4
+    }
+  }
+  if constexpr (has_tile<PREC, TB>(128, 256)) {
+    if (t.bn == 256) return launch_mfma3<PREC, TB, 64, 128>(A, B, ldb, N, C, ldc, alpha, beta, s);
+  }
+  if constexpr (has_tile<PREC, TB>(256, 128)) {
+    if (t.bm == 256) return launch_mfma3<PREC, TB, 128, 64>(A, B, ldb, N, C, ldc, alpha, beta, s);
+  }
+  return launch_mfma3<PREC, TB, 64, 64>(A, B, ldb, N, C, ldc, alpha, beta, s);
+}
+
+template <int PREC, int TB, int WM, int WN>
+void launch_mfma3_batch(dim3 grid, const BatchEntry* entries, const int64_t* tile_off, int count,
+                        int64_t per_xcd, int vec, hipStream_t s) {
+  hipLaunchKernelGGL((k_spmm_mfma3_batch<PREC, TB, WM, WN>), grid, dim3(256), 0, s, entries,
+                     tile_off, count, per_xcd, vec);
+}
+
 template <int PREC, int TB>
 bws_status launch_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
-                             int64_t total_tiles, int vec, hipStream_t s) {
+                             int64_t total_tiles, const MfmaTile& t, int vec, hipStream_t s) {
   const int64_t per_xcd = ceil_div(total_tiles, 8);
   if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
   if (total_tiles == 0) return BWS_OK;
-  hipLaunchKernelGGL((k_spmm_mfma_batch<PREC, TB>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0,
-                     s, entries, tile_off, count, per_xcd, vec);
+  if (!has_tile<PREC, TB>(t.bm, t.bn)) return BWS_EUNSUPPORTED;
+  const dim3 grid((unsigned)(8 * per_xcd));
+  bool done = false;
+  if constexpr (TB != 0 && PREC != 3) {
+    if (t.dma) {
+      if (t.bn == 256)
+        hipLaunchKernelGGL((k_spmm_mfma4_batch<PREC, 64, 128, 2, 64>), grid, dim3(256), 0, s,
+                           entries, tile_off, count, per_xcd);
+      else if (t.bm == 256)
+        hipLaunchKernelGGL((k_spmm_mfma4_batch<PREC, 128, 64, 2, 64>), grid, dim3(256), 0, s,
+                           entries, tile_off, count, per_xcd);
+      else
+        hipLaunchKernelGGL((k_spmm_mfma4_batch<PREC, 64, 64, 2, 64>), grid, dim3(256), 0, s,
+                           entries, tile_off, count, per_xcd);
+      return launch_check();
+    }
+  }
+  if constexpr (has_tile<PREC, TB>(128, 256)) {
+    if (t.bn == 256) {
+      launch_mfma3_batch<PREC, TB, 64, 128>(grid, entries, tile_off, count, per_xcd, vec, s);
+      done = true;
+    }
+  }
+  if constexpr (has_tile<PREC, TB>(256, 128)) {
+    if (!done && t.bm == 256) {
+      launch_mfma3_batch<PREC, TB, 128, 64>(grid, entries, tile_off, count, per_xcd, vec, s);
+      done = true;
+    }
+  }
+  if (!done) launch_mfma3_batch<PREC, TB, 64, 64>(grid, entries, tile_off, count, per_xcd, vec, s);
   return launch_check();
 }
 
 }  // namespace
 
+// Direct-to-LDS staging applies to 16-bit B with 16-byte aligned rows and N a
+// multiple of 8 (every 16-byte piece of a row is either inside the row or
+// clamped to its last 8 columns).
+bool dma_ok(int32_t precision, int32_t b_dtype, const void* B, int64_t ldb, int64_t N) {
+  return precision != BWS_PREC_BF16X2 && b_dtype != BWS_DTYPE_F32 && N % 8 == 0 &&
+         ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(B) % 16) == 0;
+}
+
+MfmaTile mfma_tile_for(const bws_plan* plan, int32_t precision, int32_t b_dtype, int64_t M,
+                       int64_t N) {
+  MfmaTile t;
+  const int variant = plan && plan->kernel == 2 ? plan->mfma_staging : BWS_MFMA_AUTO;
+  if (variant == BWS_MFMA_LUT) {
+    t.legacy = 1;
+    return t;
+  }
+  // direct-to-LDS unless register staging is asked for; the caller clears it
+  // when B does not allow it
+  t.dma = variant != BWS_MFMA_REGS;
+  t.nbuf = variant == BWS_MFMA_DMA3 ? 3 : 2;
+  t.ks = plan && plan->kernel == 2 && plan->k_stage == 128 ? 128 : 64;
+  int bm = plan && plan->kernel == 2 ? plan->rows_per_wg : 0;
+  int bn = plan && plan->kernel == 2 ? plan->cols_per_wg : 0;
+  if (bm == 0 || bn == 0) {
+    // the larger tile when it still gives about 1.5 workgroups per CU (two fit
+    // on a CU), else 128 x 128
+    auto tiles = [&](int64_t a, int64_t b) { return ceil_div(M, a) * ceil_div(N, b); };
+    if (tiles(128, 256) >= 384) bm = 128, bn = 256;
+    else bm = 128, bn = 128;
+  }
+  if (precision == BWS_PREC_BF16X2) {
+    bn = 128;  // two B planes
+  } else if (b_dtype == BWS_DTYPE_F32) {
+    bm = bn = 128;
+  }
+  t.bm = bm;
+  t.bn = bn;
+  return t;
+}
+
+bool mfma_tile_valid(int bm, int bn) {
+  return (bm == 0 || bm == 128 || bm == 256) && (bn == 0 || bn == 128 || bn == 256) &&
+         !(bm == 256 && bn == 256);
+}
+
 bws_status spmm_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, int count,
-                           int64_t total_tiles, int32_t b_dtype, int32_t precision, int vec,
-                           hipStream_t s) {
-#define BWS_MB(P, T) return launch_mfma_batch<P, T>(entries, tile_off, count, total_tiles, vec, s)
+                           int64_t total_tiles, const MfmaTile& t, int32_t b_dtype,
+                           int32_t precision, int vec, hipStream_t s) {
+#define BWS_MB(P, T) return launch_mfma_batch<P, T>(entries, tile_off, count, total_tiles, t, vec, s)
   if (precision == BWS_PREC_BF16) {
     if (b_dtype == BWS_DTYPE_BF16) BWS_MB(1, 1);
     if (b_dtype == BWS_DTYPE_F32) BWS_MB(1, 0);
@@ -380,16 +985,20 @@ bws_status spmm_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, i
 
 bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,
-                     hipStream_t s) {
+                     const bws_plan* plan, hipStream_t s) {
+  MfmaTile t = mfma_tile_for(plan, precision, b_dtype, A->rows, N);
+  if (!dma_ok(precision, b_dtype, B, ldb, N)) t.dma = 0;
+#define BWS_M(P, T) return launch_mfma_cfg<P, T>(t, A, B, ldb, N, C, ldc, alpha, beta, s)
   if (precision == BWS_PREC_BF16) {
-    if (b_dtype == BWS_DTYPE_BF16) return launch_mfma<1, 1>(A, B, ldb, N, C, ldc, alpha, beta, s);
-    if (b_dtype == BWS_DTYPE_F32) return launch_mfma<1, 0>(A, B, ldb, N, C, ldc, alpha, beta, s);
+    if (b_dtype == BWS_DTYPE_BF16) BWS_M(1, 1);
+    if (b_dtype == BWS_DTYPE_F32) BWS_M(1, 0);
   } else if (precision == BWS_PREC_FP16) {
-    if (b_dtype == BWS_DTYPE_F16) return launch_mfma<2, 2>(A, B, ldb, N, C, ldc, alpha, beta, s);
-    if (b_dtype == BWS_DTYPE_F32) return launch_mfma<2, 0>(A, B, ldb, N, C, ldc, alpha, beta, s);
+    if (b_dtype == BWS_DTYPE_F16) BWS_M(2, 2);
+    if (b_dtype == BWS_DTYPE_F32) BWS_M(2, 0);
   } else if (precision == BWS_PREC_BF16X2) {
-    if (b_dtype == BWS_DTYPE_F32) return launch_mfma<3, 0>(A, B, ldb, N, C, ldc, alpha, beta, s);
+    if (b_dtype == BWS_DTYPE_F32) BWS_M(3, 0);
   }
+#undef BWS_M
   return BWS_EUNSUPPORTED;
 }
 

@@ -239,17 +239,34 @@ bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64
  * defaults; bws_autotune fills one by measurement.
  *   kernel        0 = auto, 1 = vector-add rows (FP32 accumulate of set bits),
  *                 2 = MFMA tiles (16-bit precisions only)
- *   rows_per_wg   vector-add path: rows of C per workgroup (64, 128 or 256)
- *   waves_per_wg  vector-add path: 8 or 16
+ *   rows_per_wg   vector-add path: rows of C per workgroup (64, 128 or 256);
+ *                 MFMA path: rows of C per workgroup (0 = auto, 128 or 256)
+ *   waves_per_wg  vector-add path: 8 or 16 (MFMA: always 4, pass 0)
  *   k_stage       vector-add path: K rows staged per LDS buffer (32 or 64)
- *   mfma_min_bits reserved for the density classifier (0 = default) */
+ *                 (MFMA: fixed by the tile, pass 0)
+ *   mfma_min_bits reserved for the density classifier (0 = default)
+ *   cols_per_wg   MFMA path: columns of C per workgroup (0 = auto, 128 or 256;
+ *                 256 x 256 is not a tile; BF16X2 uses 128)
+ *   mfma_staging  MFMA path: how B reaches LDS (BWS_MFMA_*; AUTO = direct-to-LDS
+ *                 with 2 stages where B allows it: 16-bit B, 16-byte aligned
+ *                 rows, N % 8 == 0; otherwise register staging)
+ *   reserved      pass 0 */
+enum {
+  BWS_MFMA_AUTO = 0,
+  BWS_MFMA_LUT = 1,   /* 64 x 64 per wave, A fragments from an LDS table      */
+  BWS_MFMA_REGS = 2,  /* B loaded to registers, transposed, written to LDS    */
+  BWS_MFMA_DMA3 = 3,  /* B copied straight into LDS, 3 stages in flight       */
+  BWS_MFMA_DMA2 = 4   /* B copied straight into LDS, 2 stages (the default)   */
+};
 typedef struct {
   int32_t kernel;
   int32_t rows_per_wg;
   int32_t waves_per_wg;
   int32_t k_stage;
   int32_t mfma_min_bits;
-  int32_t reserved[3];
+  int32_t cols_per_wg;
+  int32_t mfma_staging;
+  int32_t reserved;
 } bws_plan;
 
 /* C = alpha * (A · B) + beta * C, A M x K bitmap, B K x N dense (dtype b_dtype,
@@ -321,11 +338,13 @@ typedef struct {
 #define BWS_BATCH_MAX_LAUNCH 16
 typedef struct {
   int32_t kind;          /* 1 grouped vector-add, 2 grouped MFMA, 3 per-problem launches */
-  int32_t rows_per_wg;   /* tile of kind 1                                              */
+  int32_t rows_per_wg;   /* workgroup tile rows (kinds 1, 2)                             */
+  int32_t cols_per_wg;   /* workgroup tile columns (kind 1: 256; kind 2: 128 or 256)     */
   int32_t waves_per_wg;
   int32_t k_stage;
   int32_t values;        /* kind 1: the launch's masks carry values                     */
-  int32_t vec;           /* kind 2: 16-byte B loads allowed for every problem            */
+  int32_t vec;           /* kind 2: 1 = 16-byte B loads for every problem, 2 = and B
+                            staged straight into LDS (16-bit B, N % 8 == 0)              */
   int32_t first;         /* first table entry of the launch                              */
   int32_t count;         /* entries (kinds 1, 2) or problems (kind 3)                    */
   int64_t total_tiles;   /* workgroup tiles of the launch                                */

@@ -122,6 +122,42 @@ def test_spmm_mfma(oracle, bws, dev, M, K, N, p, prec):
         assert np.array_equal(C2, C)
 
 
+TILES = {"bf16": [(128, 128), (128, 256), (256, 128)], "fp16": [(128, 128), (128, 256), (256, 128)],
+         "bf16x2": [(128, 128), (256, 128)]}
+
+
+@pytest.mark.parametrize("M,K,N,p", [(1000, 500, 300, 0.05), (257, 97, 130, 0.3),
+                                     (300, 1000, 520, 0.1), (5, 3, 2, 0.9), (512, 256, 512, 0.0),
+                                     (257, 97, 136, 0.3), (1000, 300, 264, 0.05), (33, 1000, 8, 0.2),
+                                     (64, 4096, 512, 0.01)])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x2"])
+def test_spmm_mfma_tiles(oracle, bws, dev, M, K, N, p, prec):
+    """Every MFMA workgroup tile and staging variant (register-staged B,
+    direct-to-LDS B with 2 or 3 stages, and the table-lookup 64 x 64-per-wave
+    kernel) gives bit-identical results: the K order of the accumulation is
+    the same."""
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M + 7 * N)
+    Bin = {"bf16": Bd.to(torch.bfloat16), "fp16": Bd.to(torch.float16), "bf16x2": Bd}[prec]
+    ref = bws.spmm(A, Bin, prec, plan=bws.make_plan(kernel=2)).cpu().numpy()
+    bound = {"bf16": 1e-2, "fp16": 2e-3, "bf16x2": 1e-5}[prec]
+    assert rel_err(oracle, ref, W, K, B)["max_rel_s"] <= bound
+    for bm, bn in TILES[prec]:
+        for staging, ks in (("auto", 0), ("regs", 0), ("dma3", 0), ("dma2", 0), ("dma2", 128)):
+            plan = bws.make_plan(kernel=2, rows_per_wg=bm, cols_per_wg=bn, mfma_staging=staging,
+                                 k_stage=ks)
+            got = bws.spmm(A, Bin, prec, plan=plan).cpu().numpy()
+            assert np.array_equal(got, ref), (bm, bn, staging, ks)
+    lut = bws.make_plan(kernel=2, mfma_staging="lut")
+    assert np.array_equal(bws.spmm(A, Bin, prec, plan=lut).cpu().numpy(), ref)
+
+
+def test_spmm_mfma_tile_rejects_256x256(bws, dev):
+    A = bws.pack_bitmap(torch.ones((64, 64), device=dev))
+    B = torch.ones((64, 64), device=dev, dtype=torch.bfloat16)
+    with pytest.raises(bws.BwsError):
+        bws.spmm(A, B, "bf16", plan=bws.make_plan(kernel=2, rows_per_wg=256, cols_per_wg=256))
+
+
 def test_spmm_groups_equals_spmm(oracle, bws, dev, rng):
     M, K, N, T, G = 700, 256, 200, 64, 2
     A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.05, seed=21, values=True)
