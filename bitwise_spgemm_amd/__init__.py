@@ -118,6 +118,10 @@ _SIGS = {
     "bws_transpose_bitmap": ([_P, _I64, _I64, _I64, _P, _I64, _P], _I32),
     "bws_transpose_workspace_size": ([_I64, _I64], _SZ),
     "bws_transpose_values": ([_BP, _P, _I64, _P, _P, _P, _SZ, _P], _I32),
+    "bws_convert_workspace_size": ([_I64, _I64], _SZ),
+    "bws_coo_to_bitmap": ([_P, _P, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P], _I32),
+    "bws_csr_to_bitmap": ([_P, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P], _I32),
+    "bws_bitmap_to_csr": ([_BP, _P, _P, _P, _P, _SZ, _P], _I32),
     "bws_group_tiles": ([_I64, _I64, _I32], _I64),
     "bws_group_rows": ([_BP, _I32, _I32, C.POINTER(_Groups), _P], _I32),
     "bws_group_spilled_workspace_size": ([_I64, _I64, _I32], _SZ),
@@ -375,6 +379,108 @@ def transpose(A: Bitmap, with_values: bool = True) -> Bitmap:
 # ----------------------------------------------------------------------------
 # (2) row grouping
 # ----------------------------------------------------------------------------
+def coo_to_bitmap(row_idx: torch.Tensor, col_idx: torch.Tensor, values: Optional[torch.Tensor],
+                  shape, with_values: bool = True):
+    """COO entries (int32 device tensors, any order) -> (Bitmap, n_invalid).
+    The Bitmap carries row_ptr, and values in bitmap order if given."""
+    rows, cols = shape
+    dev = row_idx.device
+    nnz = row_idx.numel()
+    ldw = max(words_for(cols), 1)
+    W = torch.empty((max(rows, 1), ldw), dtype=torch.int32, device=dev)
+    rp = torch.empty((rows + 1,), dtype=torch.int64, device=dev)
+    vout = None
+    if with_values and values is not None:
+        vout = torch.empty((max(nnz, 1),), dtype=torch.float32, device=dev)
+    bad = torch.empty((1,), dtype=torch.int64, device=dev)
+    ws = _workspace(lib().bws_convert_workspace_size(rows, cols), dev)
+    _check(lib().bws_coo_to_bitmap(_ptr(row_idx), _ptr(col_idx), _ptr(values), nnz, rows, cols,
+                                   _ptr(W), ldw, _ptr(rp), _ptr(vout), _ptr(bad), _ptr(ws),
+                                   ws.numel(), _stream(dev)), "bws_coo_to_bitmap")
+    A = Bitmap(W[:rows], rows, cols, rp, None if vout is None else vout[:nnz])
+    return A, bad
+
+
+def csr_to_bitmap(row_ptr_t: torch.Tensor, col_idx: torch.Tensor, values: Optional[torch.Tensor],
+                  shape, with_values: bool = True):
+    """CSR (int64 row pointers, int32 columns, any column order) -> (Bitmap, n_invalid)."""
+    rows, cols = shape
+    dev = col_idx.device
+    nnz = col_idx.numel()
+    ldw = max(words_for(cols), 1)
+    W = torch.empty((max(rows, 1), ldw), dtype=torch.int32, device=dev)
+    rp = torch.empty((rows + 1,), dtype=torch.int64, device=dev)
+    vout = None
+    if with_values and values is not None:
+        vout = torch.empty((max(nnz, 1),), dtype=torch.float32, device=dev)
+    bad = torch.empty((1,), dtype=torch.int64, device=dev)
+    ws = _workspace(lib().bws_convert_workspace_size(rows, cols), dev)
+    _check(lib().bws_csr_to_bitmap(_ptr(row_ptr_t), _ptr(col_idx), _ptr(values), rows, cols,
+                                   _ptr(W), ldw, _ptr(rp), _ptr(vout), _ptr(bad), _ptr(ws),
+                                   ws.numel(), _stream(dev)), "bws_csr_to_bitmap")
+    A = Bitmap(W[:rows], rows, cols, rp, None if vout is None else vout[:nnz])
+    return A, bad
+
+
+def bitmap_to_csr(A: Bitmap):
+    """Bitmap -> (row_ptr int64 [rows+1], col_idx int32 [nnz], values f32 [nnz])."""
+    dev = A.words.device
+    rp = torch.empty((A.rows + 1,), dtype=torch.int64, device=dev)
+    ws = _workspace(lib().bws_convert_workspace_size(A.rows, A.cols), dev)
+    s = A.struct()
+    _check(lib().bws_bitmap_to_csr(C.byref(s), _ptr(rp), None, None, _ptr(ws), ws.numel(),
+                                   _stream(dev)), "bws_bitmap_to_csr")
+    nnz = int(rp[-1].item())
+    col = torch.empty((max(nnz, 1),), dtype=torch.int32, device=dev)
+    vals = torch.empty((max(nnz, 1),), dtype=torch.float32, device=dev)
+    _check(lib().bws_bitmap_to_csr(C.byref(s), _ptr(rp), _ptr(col), _ptr(vals), _ptr(ws),
+                                   ws.numel(), _stream(dev)), "bws_bitmap_to_csr")
+    return rp, col[:nnz], vals[:nnz]
+
+
+def read_matrix_market(path: str):
+    """Host reader for Matrix Market coordinate files (real / integer /
+    pattern; general / symmetric / skew-symmetric).  Returns (rows, cols,
+    row_idx int32, col_idx int32, values float32), 0-based, with symmetric
+    off-diagonal entries mirrored."""
+    import numpy as np
+    with open(path, "r") as f:
+        header = f.readline().split()
+        if len(header) < 5 or header[0].lower() != "%%matrixmarket" or \
+                header[1].lower() != "matrix" or header[2].lower() != "coordinate":
+            raise ValueError(f"{path}: not a Matrix Market coordinate matrix")
+        field, sym = header[3].lower(), header[4].lower()
+        if field not in ("real", "integer", "pattern", "double"):
+            raise ValueError(f"{path}: unsupported field {field}")
+        if sym not in ("general", "symmetric", "skew-symmetric"):
+            raise ValueError(f"{path}: unsupported symmetry {sym}")
+        line = f.readline()
+        while line.startswith("%") or not line.strip():
+            line = f.readline()
+        rows, cols, nnz = (int(x) for x in line.split())
+        ncol = 2 if field == "pattern" else 3
+        data = np.loadtxt(f, dtype=np.float64, ndmin=2, max_rows=nnz)
+    if data.shape != (nnz, ncol) and not (nnz == 0):
+        raise ValueError(f"{path}: expected {nnz} entries of {ncol} fields, got {data.shape}")
+    r = data[:, 0].astype(np.int64) - 1
+    c = data[:, 1].astype(np.int64) - 1
+    v = data[:, 2].astype(np.float32) if ncol == 3 else np.ones(nnz, np.float32)
+    if sym != "general":
+        off = r != c
+        sign = -1.0 if sym == "skew-symmetric" else 1.0
+        r, c, v = (np.concatenate([r, c[off]]), np.concatenate([c, r[off]]),
+                   np.concatenate([v, (sign * v[off]).astype(np.float32)]))
+    return rows, cols, r.astype(np.int32), c.astype(np.int32), v
+
+
+def load_matrix_market(path: str, device="cuda"):
+    """Matrix Market file -> (value-carrying Bitmap on the device, n_invalid)."""
+    rows, cols, r, c, v = read_matrix_market(path)
+    dev = torch.device(device)
+    return coo_to_bitmap(torch.from_numpy(r).to(dev), torch.from_numpy(c).to(dev),
+                         torch.from_numpy(v).to(dev), (rows, cols))
+
+
 def group_tiles(rows: int, cols: int, tile_rows: int) -> int:
     return int(lib().bws_group_tiles(rows, cols, tile_rows))
 

@@ -164,6 +164,32 @@ bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
                                 int64_t* row_ptr_T, float* values_T, void* workspace,
                                 size_t workspace_bytes, bws_stream stream);
 
+/* Format converters (replace the serial tiling pass csr2tiledcsr,
+ * source/src/main.cu:499-588, and cuSPARSE DenseToSparse, main.cu:630-690).
+ * Into a bitmap, entries may come in any order: W (rows x ldw, every word
+ * written) gets one bit per entry; row_ptr [rows+1] (optional) the popcount
+ * prefix; values_out [nnz] (optional) the entries' values in the bitmap's
+ * (row, column) order.  *n_invalid (device, written by the call) counts
+ * entries outside the matrix and repeated positions; when it is nonzero the
+ * bitmap holds the valid, distinct positions and values_out is unspecified at
+ * repeated ones.  row_idx / col_idx are int32; CSR row pointers int64. */
+size_t bws_convert_workspace_size(int64_t rows, int64_t cols);
+bws_status bws_coo_to_bitmap(const int32_t* row_idx, const int32_t* col_idx, const float* values,
+                             int64_t nnz, int64_t rows, int64_t cols, uint32_t* W, int64_t ldw,
+                             int64_t* row_ptr, float* values_out, unsigned long long* n_invalid,
+                             void* workspace, size_t workspace_bytes, bws_stream stream);
+bws_status bws_csr_to_bitmap(const int64_t* csr_row_ptr, const int32_t* col_idx,
+                             const float* values, int64_t rows, int64_t cols, uint32_t* W,
+                             int64_t ldw, int64_t* row_ptr, float* values_out,
+                             unsigned long long* n_invalid, void* workspace,
+                             size_t workspace_bytes, bws_stream stream);
+/* Out of a bitmap: row_ptr [rows+1], col_idx [nnz] (ascending within a row,
+ * may be NULL) and values_out [nnz] (A.values copied, or 1.0 for a 0/1 mask;
+ * may be NULL). */
+bws_status bws_bitmap_to_csr(const bws_bitmap* A, int64_t* row_ptr, int32_t* col_idx,
+                             float* values_out, void* workspace, size_t workspace_bytes,
+                             bws_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* (2) Row grouping by bitmask                                              */
 /* ------------------------------------------------------------------------ */
