@@ -583,9 +583,12 @@ def spmm(A: Bitmap, B: torch.Tensor, precision: str = "fp32", alpha: float = 1.0
         if beta != 0.0:
             raise ValueError("beta != 0 needs `out`")
     s = A.struct()
+    pl = None if plan is None else C.byref(plan)
+    wsz = lib().bws_spmm_workspace_size(C.byref(s), N, PRECISION[precision], pl)
+    ws = _workspace(wsz, dev) if wsz else None
     _check(lib().bws_spmm(C.byref(s), _ptr(B), DTYPE[B.dtype], B.stride(0), N, _ptr(out),
-                          out.stride(0), alpha, beta, PRECISION[precision],
-                          None if plan is None else C.byref(plan), None, 0, _stream(dev)),
+                          out.stride(0), alpha, beta, PRECISION[precision], pl, _ptr(ws),
+                          0 if ws is None else ws.numel(), _stream(dev)),
            "bws_spmm")
     return out
 
@@ -599,9 +602,12 @@ def spmm_tA(AT: Bitmap, dC: torch.Tensor, precision: str = "fp32", alpha: float 
     if out is None:
         out = torch.empty((AT.rows, N), dtype=torch.float32, device=dev)
     s = AT.struct()
+    pl = None if plan is None else C.byref(plan)
+    wsz = lib().bws_spmm_workspace_size(C.byref(s), N, PRECISION[precision], pl)
+    ws = _workspace(wsz, dev) if wsz else None
     _check(lib().bws_spmm_tA(C.byref(s), _ptr(dC), DTYPE[dC.dtype], dC.stride(0), N, _ptr(out),
-                             out.stride(0), alpha, beta, PRECISION[precision],
-                             None if plan is None else C.byref(plan), None, 0, _stream(dev)),
+                             out.stride(0), alpha, beta, PRECISION[precision], pl, _ptr(ws),
+                             0 if ws is None else ws.numel(), _stream(dev)),
            "bws_spmm_tA")
     return out
 

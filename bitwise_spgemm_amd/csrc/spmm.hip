@@ -48,6 +48,10 @@ ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N) {
 bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,
                      const bws_plan* plan, hipStream_t s);
+size_t spmm_vals_workspace(const bws_bitmap* A);
+bws_status spmm_mfma_vals(const bws_bitmap* A, const void* B, int32_t precision, int64_t ldb,
+                          int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                          void* workspace, hipStream_t s);
 bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
                               int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, hipStream_t s);
@@ -124,19 +128,17 @@ extern "C" {
 
 size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision,
                                const bws_plan* plan) {
-  (void)A;
   (void)N;
-  (void)precision;
-  (void)plan;
-  return 0;
+  if (!A || A->rows < 0 || A->cols < 0 || !A->values) return 0;
+  if (precision != BWS_PREC_BF16 && precision != BWS_PREC_FP16) return 0;
+  if (plan && plan->kernel == 1) return 0;
+  return spmm_vals_workspace(A);
 }
 
 bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                     float* C, int64_t ldc, float alpha, float beta, int32_t precision,
                     const bws_plan* plan, void* workspace, size_t workspace_bytes,
                     bws_stream stream) {
-  (void)workspace;
-  (void)workspace_bytes;
   BWS_TRY(check_bitmap(A, false));
   if (N < 0 || !dtype_ok(b_dtype) || !prec_ok(precision)) return BWS_EINVAL;
   if (!combo_ok(precision, b_dtype)) return BWS_EUNSUPPORTED;
@@ -151,6 +153,15 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   if ((reinterpret_cast<uintptr_t>(C) & 3u) ||
       (reinterpret_cast<uintptr_t>(B) & (esize(b_dtype) - 1)))
     return BWS_EINVAL;
+  // A value-carrying A on MFMA needs the dense 16-bit copy of A in the
+  // workspace; without it (or for B layouts the direct-to-LDS kernel cannot
+  // take) the FP32 vector path runs, which is at least as accurate.
+  if (A->values && A->cols > 0 && (precision == BWS_PREC_BF16 || precision == BWS_PREC_FP16) &&
+      (!plan || plan->kernel != 1) && b_dtype != BWS_DTYPE_F32 &&
+      dma_ok(precision, b_dtype, B, ldb, N) && workspace &&
+      workspace_bytes >= spmm_vals_workspace(A) &&
+      (reinterpret_cast<uintptr_t>(workspace) & 255u) == 0)
+    return spmm_mfma_vals(A, B, precision, ldb, N, C, ldc, alpha, beta, workspace, stream);
   return spmm_dispatch(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, plan, stream);
 }
 

@@ -28,47 +28,17 @@
 //
 // BF16X2: B = hi + lo with hi = bf16(b), lo = bf16(b - hi); both are staged
 // and both products accumulate into the same FP32 accumulators.
-#include "bws_internal.hpp"
+#include "mfma_common.hpp"
 
 namespace bws {
 namespace {
 
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
-typedef __attribute__((ext_vector_type(4))) float f32x4;
 
 constexpr int BM = 128, BN = 128, KS = 128;
 constexpr int LDK = KS + 8;  // 16-bit elements per LDS row: 272 bytes, 16-byte aligned
 
-__device__ __forceinline__ uint32_t f32_to_bf16_rne(float x) {
-  const __bf16 h = (__bf16)x;
-  uint16_t u;
-  __builtin_memcpy(&u, &h, 2);
-  return u;
-}
-__device__ __forceinline__ uint32_t f32_to_f16_rne(float x) {
-  const _Float16 h = (_Float16)x;
-  uint16_t u;
-  __builtin_memcpy(&u, &h, 2);
-  return u;
-}
-__device__ __forceinline__ float bf16_to_f32(uint32_t u) { return __uint_as_float(u << 16); }
 
-// PREC: 1 = BF16, 2 = FP16, 3 = BF16X2.  TB: B element type 0 = f32, 1 = bf16, 2 = f16.
-template <int PREC>
-__device__ __forceinline__ f32x4 mfma(const uint4& a, const uint4& b, const f32x4& c) {
-  if constexpr (PREC == 2) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
-                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
-  } else {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-  }
-}
 
-__device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t per_xcd) {
-  return (g & 7) * per_xcd + (g >> 3);
-}
 
 // 16-bit pair (lo from a, hi from b) of element x (0..7) of two packed rows.
 __device__ __forceinline__ uint32_t pick2(const uint4& ra, const uint4& rb, int x) {
@@ -503,31 +473,6 @@ __device__ __forceinline__ void mfma3_tile(const uint32_t* __restrict__ W, int64
 // data meets zero A bits or is never stored.  B must be finite (0 x Inf = NaN
 // in a dense contraction, as in any GEMM).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void dma16(const void* g, uint32_t lds_addr) {
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
-               :
-               : "v"(g), "s"(lds_addr)
-               : "memory", "m0");
-}
-__device__ __forceinline__ void dma4(const void* g, uint32_t lds_addr) {
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off"
-               :
-               : "v"(g), "s"(lds_addr)
-               : "memory", "m0");
-}
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
-}
-__device__ __forceinline__ uint32_t lds_u32(const void* p) {
-  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
-}
-typedef short v4s __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ uint2 tr_read(uint32_t lds_addr) {
-  const v4s v = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-      (v4s __attribute__((address_space(3)))*)(uintptr_t)lds_addr);
-  return __builtin_bit_cast(uint2, v);
-}
 
 template <int WM, int WN, int NBUF, int KS_>
 struct Cfg4 {
@@ -591,9 +536,30 @@ __device__ __forceinline__ void mfma4_tile(const uint32_t* __restrict__ W, int64
     aw[t] = idx % KW;
   }
   const uint32_t sB0 = lds_u32(&sB[0][0]), sA0 = lds_u32(&sA[0][0]);
+  // stage 0 sources; a full stage only adds the stage's K offset, the last
+  // (partial) stage clamps rows and words
+  const uint16_t* bp0[NB_W];
+  const uint32_t* ap0[NA_W];
+#pragma unroll
+  for (int t = 0; t < NB_W; ++t) bp0[t] = bsrc[t] + (int64_t)brow[t] * ldb;
+#pragma unroll
+  for (int t = 0; t < NA_W; ++t) ap0[t] = asrc[t] + aw[t];
+  auto buf_of = [](int64_t st) -> int {
+    return (NBUF & (NBUF - 1)) == 0 ? (int)(st & (NBUF - 1)) : (int)(st % NBUF);
+  };
 
   auto issue = [&](int64_t st) {
-    const int buf = (int)(st % NBUF);
+    const int buf = buf_of(st);
+    if ((st + 1) * KS <= K) {
+      const int64_t boff = st * KS * ldb;
+#pragma unroll
+      for (int t = 0; t < NB_W; ++t)
+        dma16(bp0[t] + boff, sB0 + (uint32_t)(buf * SB + (wave * NB_W + t) * 1024));
+#pragma unroll
+      for (int t = 0; t < NA_W; ++t)
+        dma4(ap0[t] + st * KW, sA0 + (uint32_t)(buf * BM * KW * 4 + (wave * NA_W + t) * 256));
+      return;
+    }
 #pragma unroll
     for (int t = 0; t < NB_W; ++t) {
       int64_t k = st * KS + brow[t];
@@ -626,7 +592,7 @@ __device__ __forceinline__ void mfma4_tile(const uint32_t* __restrict__ W, int64
   const int c0 = (WN * wn) / 16;
 
   for (int64_t st = 0; st < nst; ++st) {
-    const int buf = (int)(st % NBUF);
+    const int buf = buf_of(st);
     if (st + NBUF - 2 < nst) wait_vmcnt<(NBUF - 2) * G::PER_STAGE>();
     else wait_vmcnt<0>();
     __syncthreads();
@@ -653,26 +619,52 @@ __device__ __forceinline__ void mfma4_tile(const uint32_t* __restrict__ W, int64
     }
     if (__ballot(any != 0u) != 0ull) {
       const uint32_t sbase = sB0 + (uint32_t)(buf * SB) + rd_lane;
-      uint4 bf[2][TN];
-      auto frags = [&](int ks, int d) {
+      // all B fragments of the stage are requested first (two K steps at
+      // KS = 64), then the MFMAs run as they arrive
+      constexpr int KP = KW < 2 ? KW : 2;  // K steps whose fragments are held at once
 #pragma unroll
-        for (int b = 0; b < TN; ++b) {
-          const uint32_t a0 = sbase + (uint32_t)(32 * ((c0 + b) ^ fsw)) + (uint32_t)(32 * ks * ROWB);
-          const uint2 lo = tr_read(a0);
-          const uint2 hi = tr_read(a0 + 4 * ROWB);
-          bf[d][b] = make_uint4(lo.x, lo.y, hi.x, hi.y);
-        }
-      };
-      frags(0, 0);
+      for (int k0 = 0; k0 < KW; k0 += KP) {
+        uint4 bf[KP][TN];
 #pragma unroll
-      for (int ks = 0; ks < KW; ++ks) {
-        const int d = ks & 1;
-        if (ks + 1 < KW) frags(ks + 1, d ^ 1);
+        for (int d = 0; d < KP; ++d)
 #pragma unroll
-        for (int a = 0; a < TM; ++a) {
-          const uint4 af = a_frag_bits<PREC>(wr[a][ks], q);
+          for (int b = 0; b < TN; ++b) {
+            const uint32_t a0 = sbase + (uint32_t)(32 * ((c0 + b) ^ fsw)) +
+                                (uint32_t)(32 * (k0 + d) * ROWB);
+            const uint2 lo = tr_read(a0);
+            const uint2 hi = tr_read(a0 + 4 * ROWB);
+            bf[d][b] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+          }
+        if constexpr (TN == 8) {
+          // 64 x 128 wave tile: the A fragment of the next row tile is built
+          // while this row tile's 8 MFMAs run, one MFMA per two VALU ops
+          uint4 afc = a_frag_bits<PREC>(wr[0][k0], q);
 #pragma unroll
-          for (int b = 0; b < TN; ++b) acc[a][b] = mfma<PREC>(af, bf[d][b], acc[a][b]);
+          for (int d = 0; d < KP; ++d)
+#pragma unroll
+            for (int a = 0; a < TM; ++a) {
+              const int an = (a + 1 < TM) ? a + 1 : 0;
+              const int dn = (a + 1 < TM) ? d : d + 1;
+              uint4 afn = afc;
+              if (dn < KP) afn = a_frag_bits<PREC>(wr[an][k0 + dn], q);
+#pragma unroll
+              for (int b = 0; b < TN; ++b) acc[a][b] = mfma<PREC>(afc, bf[d][b], acc[a][b]);
+#pragma unroll
+              for (int b = 0; b < TN; ++b) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two VALU
+              }
+              afc = afn;
+            }
+        } else {
+#pragma unroll
+          for (int d = 0; d < KP; ++d)
+#pragma unroll
+            for (int a = 0; a < TM; ++a) {
+              const uint4 af = a_frag_bits<PREC>(wr[a][k0 + d], q);
+#pragma unroll
+              for (int b = 0; b < TN; ++b) acc[a][b] = mfma<PREC>(af, bf[d][b], acc[a][b]);
+            }
         }
       }
     }
@@ -953,6 +945,9 @@ MfmaTile mfma_tile_for(const bws_plan* plan, int32_t precision, int32_t b_dtype,
   }
   if (precision == BWS_PREC_BF16X2) {
     bn = 128;  // two B planes
+    if (!(plan && plan->kernel == 2 && plan->rows_per_wg) &&
+        ceil_div(M, 256) * ceil_div(N, 128) >= 384)
+      bm = 256;
   } else if (b_dtype == BWS_DTYPE_F32) {
     bm = bn = 128;
   }

@@ -151,6 +151,32 @@ def test_spmm_mfma_tiles(oracle, bws, dev, M, K, N, p, prec):
     assert np.array_equal(bws.spmm(A, Bin, prec, plan=lut).cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("M,K,N,p", [(300, 500, 136, 0.1), (128, 64, 128, 0.5), (1000, 97, 264, 0.05),
+                                     (17, 1000, 8, 0.2), (1024, 1024, 512, 0.1)])
+@pytest.mark.parametrize("prec,dt", [("bf16", torch.bfloat16), ("fp16", torch.float16)])
+def test_spmm_mfma_value_mask(oracle, bws, dev, M, K, N, p, prec, dt):
+    """Value-carrying A on MFMA (dense 16-bit copy of A in the workspace):
+    exact against the FP64 product of the rounded values and rounded B, and
+    within the precision's bound of the unrounded product."""
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M + 3 * N, values=True)
+    rnd = oracle.round_bf16 if prec == "bf16" else oracle.round_f16
+    C = bws.spmm(A, Bd.to(dt), prec).cpu().numpy()
+    e = rel_err(oracle, C, W, K, rnd(B), rp, rnd(vals))
+    assert e["max_rel_s"] <= 1e-5, e
+    bound = 1e-2 if prec == "bf16" else 2e-3
+    assert rel_err(oracle, C, W, K, B, rp, vals)["max_rel_s"] <= bound
+    # without a workspace the FP32 vector path runs (same A, same B)
+    s = A.struct()
+    out = torch.empty((M, N), device=dev)
+    import ctypes
+    assert bws.lib().bws_spmm(ctypes.byref(s), ctypes.c_void_p(Bd.to(dt).data_ptr()),
+                              bws.DTYPE[dt], N, N, ctypes.c_void_p(out.data_ptr()), N, 1.0, 0.0,
+                              bws.PRECISION[prec], None, None, 0, bws._stream(dev)) == 0
+    torch.cuda.synchronize()
+    Bx = (oracle.round_bf16(B) if prec == "bf16" else oracle.round_f16(B))
+    assert rel_err(oracle, out.cpu().numpy(), W, K, Bx, rp, vals)["max_rel_s"] <= 1e-5
+
+
 def test_spmm_mfma_tile_rejects_256x256(bws, dev):
     A = bws.pack_bitmap(torch.ones((64, 64), device=dev))
     B = torch.ones((64, 64), device=dev, dtype=torch.bfloat16)
