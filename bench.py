@@ -20,7 +20,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFS = 2500.0  # dense BF16 / FP16
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "spmm_fp32_4096_clustered_traffic.json")
+# HBM bytes per launch of the benchmark kernel, from the FETCH_SIZE / WRITE_SIZE
+# counter runs summarised in profiles/ (one file per precision)
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "spmm_{}_4096_clustered_traffic.json")
 
 
 def log(*a):
@@ -155,9 +157,10 @@ def main():
     alg_bytes = 4 * M * ((K + 31) // 32) + s_b * K * N + 4 * M * N
     achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(PROFILE_TRAFFIC) and args.precision == "fp32" and not args.iid and M == N == K == 4096:
+    tfile = PROFILE_TRAFFIC.format(args.precision)
+    if os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125:
         try:
-            traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     if args.precision == "fp32":
