@@ -241,8 +241,11 @@ __device__ __forceinline__ uint32_t lds_addr_of(const void* p) {
 // m holds the stage's row mask bit-reversed (bit k <-> stage column k), so the
 // lowest set bit is the next column: s_ff1 + shift + xor on the scalar unit.
 __device__ __forceinline__ void take_bit(uint64_t& m, int& k) {
-  k = __builtin_ctzll(m);
-  m ^= 1ull << k;
+  // two scalar instructions per bit (find first set, clear it); the generic
+  // form k = ctz(m), m ^= 1 << k costs a shift and an xor instead of the
+  // clear, and this loop is bound by scalar issue
+  asm volatile("s_ff1_i32_b64 %0, %1" : "=s"(k) : "s"(m));
+  asm volatile("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(k));
 }
 
 // One output tile ; the
@@ -348,13 +351,17 @@ __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64
       }
     }
 
+    // every row's stage mask, masked and bit-reversed once in VALU (lane r
+    // holds row r), so each row below only reads its mask and its count
+    const uint32_t wo = WPS == 2 ? (uint32_t)__shfl_xor((int)wcur, 32, 64) : 0u;
+    const uint64_t mm = ((((uint64_t)wcur) << 32) | wo) & smask;
+    const uint32_t rev_lo = __brev((uint32_t)(mm >> 32)), rev_hi = __brev((uint32_t)mm);
+    const int cnt_v = __popc(rev_lo) + __popc(rev_hi);
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)wcur, r);
-      const uint32_t lo =
-          WPS == 2 ? (uint32_t)__builtin_amdgcn_readlane((int)wcur, r + 32) : 0u;
-      uint64_t m = __builtin_bitreverse64(((((uint64_t)hi) << 32) | lo) & smask);
-      int n = __popcll(m);
+      uint64_t m = (((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)rev_hi, r)) << 32) |
+                   (uint32_t)__builtin_amdgcn_readlane((int)rev_lo, r);
+      int n = __builtin_amdgcn_readlane(cnt_v, r);
       int j = 0;
       while (n >= 4) {
         int k0, k1, k2, k3;
