@@ -10,45 +10,16 @@
 // skipped as a whole.  The epilogue stores only set bits, at the value index
 // row_ptr[r] + word offset + popcount of the earlier bits of the word.
 // BF16X2 splits both operands (hi + lo) and accumulates hi·hi + hi·lo + lo·hi.
-#include "bws_internal.hpp"
+#include "mfma_common.hpp"
 
 namespace bws {
 namespace {
 
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
-typedef __attribute__((ext_vector_type(4))) float f32x4;
-
 constexpr int BR = 128, BK = 128, NSTG = 64;
 constexpr int LDN = NSTG + 8;  // 144-byte LDS rows: 16-byte aligned, conflict-free reads
 
-__device__ __forceinline__ uint32_t to_bf16(float x) {
-  const __bf16 h = (__bf16)x;
-  uint16_t u;
-  __builtin_memcpy(&u, &h, 2);
-  return u;
-}
-__device__ __forceinline__ uint32_t to_f16(float x) {
-  const _Float16 h = (_Float16)x;
-  uint16_t u;
-  __builtin_memcpy(&u, &h, 2);
-  return u;
-}
-
-template <int PREC>
-__device__ __forceinline__ f32x4 mfma(const uint4& a, const uint4& b, const f32x4& c) {
-  if constexpr (PREC == 2) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
-                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
-  } else {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-  }
-}
-
-__device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t per_xcd) {
-  return (g & 7) * per_xcd + (g >> 3);
-}
+__device__ __forceinline__ uint32_t to_bf16(float x) { return f32_to_bf16_rne(x); }
+__device__ __forceinline__ uint32_t to_f16(float x) { return f32_to_f16_rne(x); }
 
 // Load 8 consecutive elements (row, n..n+7) as raw input, zero outside.
 template <int TB>
@@ -245,6 +216,125 @@ k_sddmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
     }
 }
 
+// Direct-to-LDS variant for 16-bit dC and B (rows 16-byte aligned, leading
+// dimensions multiples of 8, N a multiple of the 64-column stage): both
+// operands are copied row-major into LDS by global_load_lds_dwordx4, 16-byte
+// chunk c of row r at chunk c ^ ((r >> 1) & 7), so the 16 lanes reading one
+// chunk of 16 rows hit 16 different bank groups; each fragment is one
+// 16-byte read.  Two stages, 64 KiB of LDS, two workgroups per CU.
+constexpr int ROWB_D = NSTG * 2;          // 128 bytes per staged row
+constexpr int STAGE_D = BR * ROWB_D;      // 16 KiB per operand per stage
+constexpr int NC_W = STAGE_D / 1024 / 4;  // copies per wave per operand per stage
+
+template <int PREC>
+__global__ void __launch_bounds__(256, 2)
+k_sddmm_mfma_dma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+                 const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ woff,
+                 const uint16_t* __restrict__ dC, int64_t lddc, const uint16_t* __restrict__ B,
+                 int64_t ldb, int64_t N, float* __restrict__ dA, int64_t row_blocks,
+                 int64_t k_blocks, int64_t per_xcd) {
+  __shared__ __attribute__((aligned(1024))) uint8_t sC[2][STAGE_D];
+  __shared__ __attribute__((aligned(1024))) uint8_t sB[2][STAGE_D];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * k_blocks) return;
+  const int64_t kbk = tile / row_blocks, rb = tile - kbk * row_blocks;
+  const int64_t r0 = rb * BR, k0 = kbk * BK;
+  const int64_t wpr = (K + 31) / 32;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int j = lane & 15, q = lane >> 4;
+
+  {  // skip an empty 128 x 128 bitmap block (all waves agree through the barrier)
+    uint32_t any = 0;
+    const int row = tid >> 1;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const int64_t w = k0 / 32 + 2 * (tid & 1) + x;
+      if (r0 + row < M && w < wpr) any |= W[(r0 + row) * ldw + w] & valid_bits(w, K);
+    }
+    if (!__syncthreads_or(any != 0u)) return;
+  }
+
+  const uint16_t* csrc[NC_W];
+  const uint16_t* bsrc[NC_W];
+#pragma unroll
+  for (int t = 0; t < NC_W; ++t) {
+    const int g = wave * NC_W + t;  // 1 KiB = 8 staged rows
+    const int r = 8 * g + lane / 8;
+    const int c = (lane & 7) ^ ((r >> 1) & 7);
+    int64_t rc = r0 + r, rk = k0 + r;
+    if (rc > M - 1) rc = M - 1;
+    if (rk > K - 1) rk = K - 1;
+    csrc[t] = dC + rc * lddc + 8 * c;
+    bsrc[t] = B + rk * ldb + 8 * c;
+  }
+  const uint32_t sC0 = lds_u32(&sC[0][0]), sB0 = lds_u32(&sB[0][0]);
+  auto issue = [&](int64_t st) {
+    const int buf = (int)(st & 1);
+#pragma unroll
+    for (int t = 0; t < NC_W; ++t) {
+      const uint32_t off = (uint32_t)(buf * STAGE_D + (wave * NC_W + t) * 1024);
+      dma16(csrc[t] + st * NSTG, sC0 + off);
+      dma16(bsrc[t] + st * NSTG, sB0 + off);
+    }
+  };
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int64_t nst = N / NSTG;
+  issue(0);
+  for (int64_t st = 0; st < nst; ++st) {
+    const int buf = (int)(st & 1);
+    wait_vmcnt<0>();
+    __syncthreads();
+    if (st + 1 < nst) issue(st + 1);
+#pragma unroll
+    for (int ks = 0; ks < NSTG / 32; ++ks) {
+      uint4 af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int r = 64 * wm + 16 * a + j;
+        af[a] = *reinterpret_cast<const uint4*>(
+            &sC[buf][r * ROWB_D + 16 * ((4 * ks + q) ^ ((r >> 1) & 7))]);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int r = 64 * wn + 16 * b + j;
+        bf[b] = *reinterpret_cast<const uint4*>(
+            &sB[buf][r * ROWB_D + 16 * ((4 * ks + q) ^ ((r >> 1) & 7))]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = mfma<PREC>(af[a], bf[b], acc[a][b]);
+    }
+  }
+
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = r0 + 64 * wm + 16 * a + 4 * q + e;
+      if (r >= M) continue;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int64_t k = k0 + 64 * wn + 16 * b + j;
+        if (k >= K) continue;
+        const int64_t w = k >> 5;
+        const uint32_t word = W[r * ldw + w];
+        const int i = (int)(k & 31);
+        if (word & wbit(i))
+          dA[row_ptr[r] + woff[r * wpr + w] + rank_before(word, i)] = acc[a][b][e];
+      }
+    }
+}
+
 template <int PREC, int TB>
 bws_status launch_sddmm(const bws_bitmap* A, const int32_t* woff, const void* dC, int64_t lddc,
                         const void* B, int64_t ldb, int64_t N, float* dA, hipStream_t s) {
@@ -273,6 +363,28 @@ bws_status sddmm_mfma_launch(const bws_bitmap* A, const void* dC, const void* B,
                              void* ws, hipStream_t s) {
   int32_t* woff = static_cast<int32_t*>(ws);
   BWS_TRY(launch_word_offsets(A->words, A->rows, A->cols, A->ldw, woff, s));
+  const bool dma = dtype != BWS_DTYPE_F32 && precision != BWS_PREC_BF16X2 && N >= NSTG &&
+                   N % NSTG == 0 &&
+                   lddc % 8 == 0 && ldb % 8 == 0 && reinterpret_cast<uintptr_t>(dC) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(B) % 16 == 0;
+  if (dma && ((precision == BWS_PREC_BF16 && dtype == BWS_DTYPE_BF16) ||
+              (precision == BWS_PREC_FP16 && dtype == BWS_DTYPE_F16))) {
+    const int64_t row_blocks = ceil_div(A->rows, BR), k_blocks = ceil_div(A->cols, BK);
+    const int64_t per_xcd = ceil_div(row_blocks * k_blocks, 8);
+    if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+    const dim3 grid((unsigned)(8 * per_xcd));
+    const uint16_t* c16 = static_cast<const uint16_t*>(dC);
+    const uint16_t* b16 = static_cast<const uint16_t*>(B);
+    if (precision == BWS_PREC_FP16)
+      hipLaunchKernelGGL(k_sddmm_mfma_dma<2>, grid, dim3(256), 0, s, A->words, A->rows, A->cols,
+                         A->ldw, A->row_ptr, woff, c16, lddc, b16, ldb, N, dA, row_blocks,
+                         k_blocks, per_xcd);
+    else
+      hipLaunchKernelGGL(k_sddmm_mfma_dma<1>, grid, dim3(256), 0, s, A->words, A->rows, A->cols,
+                         A->ldw, A->row_ptr, woff, c16, lddc, b16, ldb, N, dA, row_blocks,
+                         k_blocks, per_xcd);
+    return launch_check();
+  }
   if (precision == BWS_PREC_BF16) {
     if (dtype == BWS_DTYPE_BF16) return launch_sddmm<1, 1>(A, woff, dC, lddc, B, ldb, N, dA, s);
     if (dtype == BWS_DTYPE_F32) return launch_sddmm<1, 0>(A, woff, dC, lddc, B, ldb, N, dA, s);

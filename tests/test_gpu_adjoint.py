@@ -20,7 +20,9 @@ def sddmm_err(oracle, W, K, rp, dC, B, got):
 
 
 @pytest.mark.parametrize("M,K,N,p", [(128, 128, 64, 0.1), (300, 500, 200, 0.05), (17, 33, 3, 0.5),
-                                     (1024, 512, 1000, 0.02), (64, 64, 32, 1.0)])
+                                     (1024, 512, 1000, 0.02), (64, 64, 32, 1.0),
+                                     (1000, 300, 256, 0.05), (129, 1000, 64, 0.3),
+                                     (512, 512, 2048, 0.1)])
 @pytest.mark.parametrize("prec,dt", [("bf16", torch.bfloat16), ("bf16", torch.float32),
                                      ("fp16", torch.float16), ("fp16", torch.float32),
                                      ("bf16x2", torch.float32)])
