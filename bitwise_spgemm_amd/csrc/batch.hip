@@ -101,6 +101,7 @@ Choice classify(const bws_problem& p, int32_t b_dtype, int32_t precision, const 
     const size_t esz = b_dtype == BWS_DTYPE_F32 ? 4 : 2;
     c.vec = (p.ldb % (int64_t)(16 / esz) == 0) && (reinterpret_cast<uintptr_t>(p.B) % 16 == 0);
     c.tile.dma = c.tile.dma && dma_ok(precision, b_dtype, p.B, p.ldb, p.N);
+    if (!c.tile.dma && c.tile.bm == 256 && c.tile.bn == 256) c.tile.bm = 128;
     return c;
   }
   if (p.A.cols > 0 && valu_direct_ok(b_dtype, p.B, p.ldb, p.N, p.C, p.ldc)) {
@@ -314,7 +315,7 @@ bws_status bws_autotune(const bws_bitmap* A, int64_t nnz, const void* B, int32_t
     cand.push_back(p);
   }
   if (precision != BWS_PREC_FP32 && !A->values) {
-    static const int32_t tiles[][2] = {{128, 256}, {256, 128}, {128, 128}};
+    static const int32_t tiles[][2] = {{256, 256}, {128, 256}, {256, 128}, {128, 128}};
     for (const auto& t : tiles) {
       if (precision == BWS_PREC_BF16X2 && t[1] == 256) continue;
       bws_plan p{};

@@ -79,6 +79,14 @@ struct BatchEntry {
   int64_t row_blocks, col_blocks;
 };
 
+// the 256 x 256-tile direct-to-LDS kernel (spmm_mfma_big.hip)
+bws_status launch_mfma4_big(int32_t precision, const bws_bitmap* A, const void* B, int64_t ldb,
+                            int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                            hipStream_t s);
+bws_status launch_mfma4_big_batch(int32_t precision, const BatchEntry* entries,
+                                  const int64_t* tile_off, int count, int64_t total_tiles,
+                                  hipStream_t s);
+
 // Device scratch carving: a linear allocator over the caller's workspace.
 struct Carve {
   char* base;

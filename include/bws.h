@@ -272,7 +272,8 @@ bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64
  *                 (MFMA: fixed by the tile, pass 0)
  *   mfma_min_bits reserved for the density classifier (0 = default)
  *   cols_per_wg   MFMA path: columns of C per workgroup (0 = auto, 128 or 256;
- *                 256 x 256 is not a tile; BF16X2 uses 128)
+ *                 256 x 256 only with the direct-to-LDS staging of 16-bit B,
+ *                 else it runs as 128 x 256; BF16X2 uses 128)
  *   mfma_staging  MFMA path: how B reaches LDS (BWS_MFMA_*; AUTO = direct-to-LDS
  *                 with 2 stages where B allows it: 16-bit B, 16-byte aligned
  *                 rows, N % 8 == 0; otherwise register staging)

@@ -122,7 +122,8 @@ def test_spmm_mfma(oracle, bws, dev, M, K, N, p, prec):
         assert np.array_equal(C2, C)
 
 
-TILES = {"bf16": [(128, 128), (128, 256), (256, 128)], "fp16": [(128, 128), (128, 256), (256, 128)],
+TILES = {"bf16": [(128, 128), (128, 256), (256, 128), (256, 256)],
+         "fp16": [(128, 128), (128, 256), (256, 128), (256, 256)],
          "bf16x2": [(128, 128), (256, 128)]}
 
 
@@ -177,11 +178,11 @@ def test_spmm_mfma_value_mask(oracle, bws, dev, M, K, N, p, prec, dt):
     assert rel_err(oracle, out.cpu().numpy(), W, K, Bx, rp, vals)["max_rel_s"] <= 1e-5
 
 
-def test_spmm_mfma_tile_rejects_256x256(bws, dev):
+def test_spmm_mfma_tile_rejects_bad_tile(bws, dev):
     A = bws.pack_bitmap(torch.ones((64, 64), device=dev))
     B = torch.ones((64, 64), device=dev, dtype=torch.bfloat16)
     with pytest.raises(bws.BwsError):
-        bws.spmm(A, B, "bf16", plan=bws.make_plan(kernel=2, rows_per_wg=256, cols_per_wg=256))
+        bws.spmm(A, B, "bf16", plan=bws.make_plan(kernel=2, rows_per_wg=512, cols_per_wg=256))
 
 
 def test_spmm_groups_equals_spmm(oracle, bws, dev, rng):
