@@ -128,6 +128,9 @@ _SIGS = {
     "bws_group_spilled_csr": ([_BP, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P], _I32),
     "bws_group_scatter": ([_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _I64, _P], _I32),
     "bws_group_gather": ([_P, _I64, _P, _I64, _I64, _I64, _P, _I32, _I32, _P, _P], _I32),
+    "bws_cluster_workspace_size": ([_I64, _I32], _SZ),
+    "bws_cluster_rows": ([_BP, _I32, _P, _P, _P, _SZ, _P], _I32),
+    "bws_spmm_clustered": ([_BP, _P, _P, _P, _I64, _I64, _P, _I64, _F, _F, _P], _I32),
     "bws_spmm_workspace_size": ([_BP, _I64, _I32, C.POINTER(_Plan)], _SZ),
     "bws_spmm": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P, _SZ,
                   _P], _I32),
@@ -557,6 +560,54 @@ def group_gather(D: torch.Tensor, A: Bitmap, groups: dict, tile_rows: int, max_g
                                   _ptr(groups["group_id"]), tile_rows, max_groups, _ptr(gv),
                                   _stream(dev)), "bws_group_gather")
     return gv
+
+
+SLOT_DELTA = 1 << 30
+
+
+class RowClusters:
+    """Row-clustering plan of a 0/1 bitmap (bws_cluster_rows): slot_row int32
+    [rows] (row | SLOT_DELTA for rows summed as a difference from their
+    chunk) and chunk_mask int32 [ceil(rows/16), ldw] (the chunks' majority
+    bits, uint32 words held as int32)."""
+
+    def __init__(self, slot_row: torch.Tensor, chunk_mask: torch.Tensor, rows: int, cols: int,
+                 max_clusters: int):
+        self.slot_row = slot_row
+        self.chunk_mask = chunk_mask
+        self.rows, self.cols, self.max_clusters = rows, cols, max_clusters
+
+
+def cluster_rows(A: Bitmap, max_clusters: int = 0) -> RowClusters:
+    """Row clustering for spmm_clustered (0/1 masks; max_clusters 0 = rows / 64)."""
+    dev = A.words.device
+    slot_row = torch.empty(max(A.rows, 1), dtype=torch.int32, device=dev)[:A.rows]
+    chunk_mask = torch.empty(((A.rows + 15) // 16, A.ldw), dtype=torch.int32, device=dev)
+    ws = _workspace(lib().bws_cluster_workspace_size(A.rows, max_clusters), dev)
+    s = A.struct()
+    _check(lib().bws_cluster_rows(C.byref(s), max_clusters, _ptr(slot_row), _ptr(chunk_mask),
+                                  _ptr(ws), ws.numel(), _stream(dev)), "bws_cluster_rows")
+    return RowClusters(slot_row, chunk_mask, A.rows, A.cols, max_clusters)
+
+
+def spmm_clustered(A: Bitmap, clusters: RowClusters, B: torch.Tensor, alpha: float = 1.0,
+                   beta: float = 0.0, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """C = alpha * A·B + beta * C through the row-clustering plan (FP32 B)."""
+    _require_gpu(B)
+    assert B.dim() == 2 and B.dtype == torch.float32 and B.shape[0] == A.cols
+    assert (clusters.rows, clusters.cols) == (A.rows, A.cols)
+    N = B.shape[1]
+    dev = B.device
+    if out is None:
+        if beta != 0.0:
+            raise ValueError("beta != 0 needs `out`")
+        out = torch.empty((A.rows, N), dtype=torch.float32, device=dev)
+    s = A.struct()
+    _check(lib().bws_spmm_clustered(C.byref(s), _ptr(clusters.slot_row),
+                                    _ptr(clusters.chunk_mask), _ptr(B), B.stride(0), N,
+                                    _ptr(out), out.stride(0), alpha, beta, _stream(dev)),
+           "bws_spmm_clustered")
+    return out
 
 
 # ----------------------------------------------------------------------------

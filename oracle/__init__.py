@@ -48,11 +48,13 @@ _SIGS = {
     "oref_sddmm_f64": [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _I64, _I64, _P, _I32],
     "oref_check_dense_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I32],
     "oref_err_metrics": [_P, _I64, _P, _P, _I64, _I64, _P],
+    "oref_cluster_leaders": [_I64, _I32],
+    "oref_cluster_rows": [_P, _I64, _I64, _I64, _I32, _P, _P],
     "oref_round_bf16_array": [_P, _I64, _P],
     "oref_round_f16_array": [_P, _I64, _P],
     "oref_split_bf16x2": [_P, _I64, _P, _P],
 }
-_RET = {"oref_group_tiles": _I64}
+_RET = {"oref_group_tiles": _I64, "oref_cluster_leaders": _I32}
 
 
 def build() -> str:
@@ -214,6 +216,22 @@ def group_rows(W, cols, tile_rows=2048, max_groups=2, rp=None, vals=None):
                                "spill_cnt", "spill_nnz", "spill_table", "spill_rev",
                                "grouped_nnz")])
     return out
+
+
+def cluster_leaders(rows, max_clusters=0):
+    return int(lib().oref_cluster_leaders(rows, max_clusters))
+
+
+def cluster_rows(W, cols, max_clusters=0):
+    """Row-clustering plan: (slot_row int32 [rows], chunk_mask uint32 [ceil(rows/16), ldw])."""
+    W = _c(W, np.uint32)
+    rows, ldw = W.shape
+    slot_row = np.empty(rows, np.int32)
+    chunk_mask = np.empty(((rows + 15) // 16, ldw), np.uint32)
+    if rows:
+        lib().oref_cluster_rows(_p(W), rows, cols, ldw, max_clusters, _p(slot_row),
+                                _p(chunk_mask))
+    return slot_row, chunk_mask
 
 
 def group_spilled_csr(W, cols, tile_rows, groups, rp=None, vals=None):

@@ -15,6 +15,7 @@
 #include "bws_oracle.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define WBIT(i) (0x80000000u >> (i)) /* column i of a word <-> bit 31-i */
@@ -320,6 +321,102 @@ void oref_symbolic_or(const uint32_t* WA, int64_t M, int64_t K, int64_t ldwa,
       if (WA[r * ldwa + k / 32] & WBIT(k % 32))
         for (int64_t s = 0; s < wpn; ++s) WC[r * ldwc + s] |= WB[k * ldwb + s];
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* row clustering                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Bits of word w that are columns < cols (MSB-first). */
+static uint32_t valid_word(int64_t w, int64_t cols) {
+  const int64_t rem = cols - 32 * w;
+  return rem >= 32 ? 0xFFFFFFFFu : (rem <= 0 ? 0u : (0xFFFFFFFFu << (32 - rem)));
+}
+
+/* Leaders used for `rows` rows: max_clusters, or rows / 64 when it is 0;
+ * at least 1 and at most rows and 65536. */
+int32_t oref_cluster_leaders(int64_t rows, int32_t max_clusters) {
+  if (rows <= 0) return 0;
+  int64_t L = max_clusters > 0 ? max_clusters : rows / 64;
+  if (L < 1) L = 1;
+  if (L > rows) L = rows;
+  if (L > 65536) L = 65536;
+  return (int32_t)L;
+}
+
+/* The row-clustering plan, serially.  There is no reference function to
+ * follow: the reference groups rows by bitmask (generate_groups_large,
+ * source/src/main.cu:50-247; the row-to-group intent of backup/exp.cu:49-88)
+ * so that a group's rows share work; this plan groups similar rows so that a
+ * chunk's shared bits are summed once.
+ *   1. Farthest-point leaders over Hamming distance: leader 0 is row 0; after
+ *      every row has taken its distance to leader i (keeping the earliest
+ *      leader on ties), leader i+1 is the row with the largest distance to
+ *      its nearest leader (lowest row on ties).
+ *   2. Rows ordered by nearest leader, ascending row within a leader.
+ *   3. Chunk c = slots 16c .. 16c+15 (fewer in the last); chunk_mask bit k is
+ *      set iff 2 * (the chunk's rows with bit k) > (rows in the chunk) + 1.
+ *   4. slot_row[s] = row | (1 << 30) when popcount(row ^ chunk_mask) <
+ *      popcount(row) (the row is then summed as a delta from the chunk). */
+void oref_cluster_rows(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
+                       int32_t max_clusters, int32_t* slot_row, uint32_t* chunk_mask) {
+  if (rows <= 0) return;
+  const int64_t wpr = words_for(cols);
+  const int32_t L = oref_cluster_leaders(rows, max_clusters);
+  int64_t* dist = (int64_t*)malloc(sizeof(int64_t) * rows);
+  int32_t* owner = (int32_t*)malloc(sizeof(int32_t) * rows);
+  int64_t* start = (int64_t*)calloc((size_t)L + 1, sizeof(int64_t));
+  int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * rows);
+  int64_t lead = 0;
+  for (int32_t it = 0; it < L; ++it) {
+    const uint32_t* lw = W + lead * ldw;
+    for (int64_t r = 0; r < rows; ++r) {
+      int64_t d = 0;
+      for (int64_t w = 0; w < wpr; ++w)
+        d += popc32((W[r * ldw + w] ^ lw[w]) & valid_word(w, cols));
+      if (it == 0 || d < dist[r]) {
+        dist[r] = d;
+        owner[r] = it;
+      }
+    }
+    int64_t best = 0;
+    for (int64_t r = 1; r < rows; ++r)
+      if (dist[r] > dist[best]) best = r;
+    lead = best;
+  }
+  for (int64_t r = 0; r < rows; ++r) start[owner[r] + 1]++;
+  for (int32_t o = 0; o < L; ++o) start[o + 1] += start[o];
+  for (int64_t r = 0; r < rows; ++r) perm[start[owner[r]]++] = (int32_t)r;
+  const int64_t chunks = (rows + 15) / 16;
+  for (int64_t c = 0; c < chunks; ++c) {
+    const int64_t n = rows - 16 * c < 16 ? rows - 16 * c : 16;
+    for (int64_t w = 0; w < ldw; ++w) {
+      uint32_t m = 0u;
+      if (w < wpr) {
+        const uint32_t vb = valid_word(w, cols);
+        for (int b = 0; b < 32; ++b) {
+          int64_t cnt = 0;
+          for (int64_t i = 0; i < n; ++i) cnt += (W[(int64_t)perm[16 * c + i] * ldw + w] & vb & WBIT(b)) != 0;
+          if (2 * cnt > n + 1) m |= WBIT(b);
+        }
+      }
+      chunk_mask[c * ldw + w] = m;
+    }
+  }
+  for (int64_t s = 0; s < rows; ++s) {
+    const int64_t r = perm[s], c = s / 16;
+    int64_t pa = 0, pd = 0;
+    for (int64_t w = 0; w < wpr; ++w) {
+      const uint32_t a = W[r * ldw + w] & valid_word(w, cols);
+      pa += popc32(a);
+      pd += popc32(a ^ chunk_mask[c * ldw + w]);
+    }
+    slot_row[s] = (int32_t)r | (pd < pa ? (int32_t)(1 << 30) : 0);
+  }
+  free(dist);
+  free(owner);
+  free(start);
+  free(perm);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -142,6 +142,33 @@ def test_sddmm_and_groups_validation(bws):
     assert L.bws_batch_prepare(arr, 1, 0, 0, None, FAKE, ws, C.byref(b)) == bws.BWS_EINVAL
 
 
+def test_cluster_validation(bws):
+    L = bws.lib()
+    a = _bm(64, 64, 2)
+    ws = L.bws_cluster_workspace_size(64, 0)
+    assert ws > 5 * 64 * 4 and L.bws_cluster_workspace_size(64, -1) == 0
+    assert L.bws_cluster_rows(C.byref(_bm(64, 64, 2, row_ptr=FAKE, values=FAKE)), 0, FAKE, FAKE,
+                              FAKE, ws, None) == bws.BWS_EUNSUPPORTED
+    assert L.bws_cluster_rows(C.byref(a), -1, FAKE, FAKE, FAKE, ws, None) == bws.BWS_EINVAL
+    assert L.bws_cluster_rows(C.byref(a), 0, None, FAKE, FAKE, ws, None) == bws.BWS_EINVAL
+    assert L.bws_cluster_rows(C.byref(a), 0, FAKE, FAKE, FAKE, ws - 1, None) == bws.BWS_EWORKSPACE
+    assert L.bws_cluster_rows(C.byref(_bm(0, 64, 2)), 0, None, None, None, 0, None) == bws.BWS_OK
+    assert L.bws_cluster_rows(C.byref(_bm(1 << 30, 64, 2)), 0, FAKE, FAKE, FAKE, ws,
+                              None) == bws.BWS_ECAPACITY
+
+    def call(A=a, sr=FAKE, B=FAKE, ldb=64, N=64, Cp=FAKE, ldc=64):
+        return L.bws_spmm_clustered(C.byref(A), sr, FAKE, B, ldb, N, Cp, ldc, 1.0, 0.0, None)
+    assert call(A=_bm(64, 64, 2, row_ptr=FAKE, values=FAKE)) == bws.BWS_EUNSUPPORTED
+    assert call(ldc=63) == bws.BWS_EINVAL
+    assert call(ldb=63) == bws.BWS_EINVAL
+    assert call(sr=None) == bws.BWS_EINVAL
+    assert call(Cp=None) == bws.BWS_EINVAL
+    assert call(N=6, ldb=6, ldc=6) == bws.BWS_EUNSUPPORTED   # N % 4
+    assert call(B=FAKE + 4) == bws.BWS_EUNSUPPORTED           # B not 16-byte aligned
+    assert call(N=0) == bws.BWS_OK
+    assert call(A=_bm(0, 64, 2)) == bws.BWS_OK
+
+
 def test_autotune_cache_host(bws, tmp_path):
     L = bws.lib()
     assert L.bws_autotune_clear() == bws.BWS_OK

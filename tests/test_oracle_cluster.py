@@ -1,0 +1,146 @@
+"""Pins the oracle's row-clustering plan (CPU only).
+
+The plan has no reference counterpart (it is this library's analysis step for
+the FP32 product), so it is pinned by what fixes it:
+1. a hand-worked case whose leaders, order and masks follow from the rules;
+2. exact clusters: rows that are copies of a few patterns must come out in
+   chunks of identical rows, each chunk mask equal to its pattern and every
+   row a zero-difference delta row;
+3. outputs checked from first principles on random masks: slot_row is a
+   permutation, each chunk mask is its chunk's majority, each flag follows
+   the popcount rule, and the leader order is farthest-point;
+4. the algebra the clustered product relies on: T + (row & ~mask)·B -
+   (mask & ~row)·B equals row·B, evaluated in FP64 against the FP64 oracle."""
+import numpy as np
+import pytest
+
+DELTA = 1 << 30
+
+
+def words(bits):
+    """Bool matrix [rows, cols] -> MSB-first uint32 words."""
+    rows, cols = bits.shape
+    wpr = (cols + 31) // 32
+    pad = np.zeros((rows, wpr * 32), bool)
+    pad[:, :cols] = bits
+    return np.packbits(pad.reshape(rows, wpr, 4, 8), axis=-1, bitorder="big").reshape(
+        rows, wpr, 4).astype(np.uint32) @ np.array([1 << 24, 1 << 16, 1 << 8, 1], np.uint32)
+
+
+def bits_of(W, cols):
+    rows = W.shape[0]
+    b = np.unpackbits(W.astype(">u4").view(np.uint8).reshape(rows, -1), axis=1)
+    return b[:, :cols].astype(bool)
+
+
+def test_words_helper_layout():
+    b = np.zeros((1, 40), bool)
+    b[0, 0] = b[0, 33] = True
+    W = words(b)
+    assert W[0, 0] == 0x80000000 and W[0, 1] == 0x40000000
+    assert (bits_of(W, 40) == b).all()
+
+
+def test_hand_worked_order(oracle):
+    # A = 0xFFFF0000, A' = A | col 31, B = 0x0000FFFF, B' = B & ~col 31, rows
+    # [A, B', A', B]: leader 0 = row 0; distances 0, 31, 1, 32 -> leader 1 =
+    # row 3; row 1 is 1 from B (31 from A), so owners [0, 1, 0, 1] and the
+    # order is rows 0, 2, 1, 3.  One chunk of 4 rows: no column is set in 3 of
+    # them, so the mask is empty and no row is a delta row.
+    W = np.array([[0xFFFF0000], [0x0000FFFE], [0xFFFF0001], [0x0000FFFF]], np.uint32)
+    slot, cm = oracle.cluster_rows(W, 32, max_clusters=2)
+    assert slot.tolist() == [0, 2, 1, 3]
+    assert cm.tolist() == [[0]]
+    # one leader: every row belongs to row 0, order is ascending rows
+    slot, _ = oracle.cluster_rows(W, 32, max_clusters=1)
+    assert slot.tolist() == [0, 1, 2, 3]
+
+
+def test_hand_worked_majority(oracle):
+    # 3 copies of P and one empty row in one chunk (n = 4): bits of P are in 3
+    # rows, 2*3 > 5, so the mask is P; P rows are delta rows (0 < popcount P),
+    # the empty row is not (popcount(P) > 0).
+    P = 0x0F0F00F1
+    W = np.array([[P], [0], [P], [P]], np.uint32)
+    slot, cm = oracle.cluster_rows(W, 32, max_clusters=1)
+    assert cm.tolist() == [[P]]
+    assert slot.tolist() == [0 | DELTA, 1, 2 | DELTA, 3 | DELTA]
+
+
+@pytest.mark.parametrize("patterns,reps,cols", [(4, 32, 200), (7, 16, 96), (3, 48, 1000)])
+def test_exact_clusters(oracle, rng, patterns, reps, cols):
+    P = rng.random((patterns, cols)) < 0.3
+    idx = rng.permutation(np.repeat(np.arange(patterns), reps))
+    W = words(P[idx])
+    slot, cm = oracle.cluster_rows(W, cols, max_clusters=patterns)
+    rows = slot & (DELTA - 1)
+    assert sorted(rows.tolist()) == list(range(len(idx)))
+    assert ((slot & DELTA) != 0).all()
+    for c in range(len(idx) // 16):
+        pats = idx[rows[16 * c:16 * c + 16]]
+        assert len(set(pats.tolist())) == 1, "a chunk mixes clusters"
+        assert (cm[c] == words(P[pats[:1]])[0]).all()
+
+
+@pytest.mark.parametrize("rows,cols,L,p", [(37, 45, 0, 0.2), (100, 64, 5, 0.1), (333, 129, 0, 0.05),
+                                           (16, 300, 16, 0.3), (1, 10, 0, 0.5), (50, 33, 50, 0.4)])
+def test_first_principles(oracle, rng, rows, cols, L, p):
+    b = rng.random((rows, cols)) < p
+    b[rng.random(rows) < 0.1] = False  # some empty rows
+    W = words(b)
+    slot, cm = oracle.cluster_rows(W, cols, max_clusters=L)
+    order = slot & (DELTA - 1)
+    assert sorted(order.tolist()) == list(range(rows))
+    nl = oracle.cluster_leaders(rows, L)
+    # farthest-point leaders and nearest-leader owners, restated from the rules
+    d = lambda r, s: int((b[r] ^ b[s]).sum())
+    dist = [d(r, 0) for r in range(rows)]
+    owner = [0] * rows
+    lead = int(np.argmax(dist))
+    for i in range(1, nl):
+        for r in range(rows):
+            if d(r, lead) < dist[r]:
+                dist[r], owner[r] = d(r, lead), i
+        lead = int(np.argmax(dist))
+    expect = sorted(range(rows), key=lambda r: (owner[r], r))
+    assert order.tolist() == expect
+    for c in range((rows + 15) // 16):
+        rr = order[16 * c:16 * c + 16]
+        cnt = b[rr].sum(0)
+        maj = 2 * cnt > len(rr) + 1
+        assert (bits_of(cm[c:c + 1], cols)[0] == maj).all()
+        assert (cm[c] & ~words(np.ones((1, cols), bool))[0] == 0).all()
+    for s in range(rows):
+        r = order[s]
+        mask = bits_of(cm[s // 16:s // 16 + 1], cols)[0]
+        is_delta = (b[r] ^ mask).sum() < b[r].sum()
+        assert bool(slot[s] & DELTA) == bool(is_delta)
+
+
+def test_delta_algebra_fp64(oracle, bws):
+    M, K, N = 300, 200, 40
+    W = bws.gen_bitmap_clustered(3, M, K, 8, 0.2, 0.03)
+    B = bws.gen_f32(4, (K, N), "normal", 0.0, 1.0)
+    slot, cm = oracle.cluster_rows(W, K, max_clusters=8)
+    assert ((slot & DELTA) != 0).mean() > 0.8  # the clusters are found
+    R = oracle.spmm_f64(W, K, B)
+    Cp = np.zeros((M, N))
+    T = oracle.spmm_f64(cm, K, B)  # chunk sums, [chunks, N]
+    for s in range(M):
+        r = slot[s] & (DELTA - 1)
+        if slot[s] & DELTA:
+            add = W[r] & ~cm[s // 16]
+            sub = cm[s // 16] & ~W[r]
+            Cp[r] = (T[s // 16] + oracle.spmm_f64(add[None], K, B)[0]
+                     - oracle.spmm_f64(sub[None], K, B)[0])
+        else:
+            Cp[r] = oracle.spmm_f64(W[r:r + 1], K, B)[0]
+    assert np.abs(Cp - R).max() <= 1e-12 * max(1.0, np.abs(R).max())
+
+
+def test_leader_count(oracle):
+    assert oracle.cluster_leaders(0, 0) == 0
+    assert oracle.cluster_leaders(10, 0) == 1
+    assert oracle.cluster_leaders(4096, 0) == 64
+    assert oracle.cluster_leaders(100, 500) == 100
+    assert oracle.cluster_leaders(10 ** 7, 10 ** 6) == 65536
