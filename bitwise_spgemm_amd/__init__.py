@@ -563,13 +563,15 @@ def group_gather(D: torch.Tensor, A: Bitmap, groups: dict, tile_rows: int, max_g
 
 
 SLOT_DELTA = 1 << 30
+SLOT_BASE1 = 1 << 29
 
 
 class RowClusters:
     """Row-clustering plan of a 0/1 bitmap (bws_cluster_rows): slot_row int32
-    [rows] (row | SLOT_DELTA for rows summed as a difference from their
-    chunk) and chunk_mask int32 [ceil(rows/16), ldw] (the chunks' majority
-    bits, uint32 words held as int32)."""
+    [rows] (row | SLOT_DELTA for rows summed as a difference from a chunk
+    base, | SLOT_BASE1 when that is the chunk's second base) and chunk_mask
+    int32 [ceil(rows/16), 2, ldw] (the chunks' two bases, uint32 words held as
+    int32)."""
 
     def __init__(self, slot_row: torch.Tensor, chunk_mask: torch.Tensor, rows: int, cols: int,
                  max_clusters: int):
@@ -582,7 +584,7 @@ def cluster_rows(A: Bitmap, max_clusters: int = 0) -> RowClusters:
     """Row clustering for spmm_clustered (0/1 masks; max_clusters 0 = rows / 64)."""
     dev = A.words.device
     slot_row = torch.empty(max(A.rows, 1), dtype=torch.int32, device=dev)[:A.rows]
-    chunk_mask = torch.empty(((A.rows + 15) // 16, A.ldw), dtype=torch.int32, device=dev)
+    chunk_mask = torch.empty(((A.rows + 15) // 16, 2, A.ldw), dtype=torch.int32, device=dev)
     ws = _workspace(lib().bws_cluster_workspace_size(A.rows, max_clusters), dev)
     s = A.struct()
     _check(lib().bws_cluster_rows(C.byref(s), max_clusters, _ptr(slot_row), _ptr(chunk_mask),

@@ -3,9 +3,10 @@
 // The reference groups rows by bitmask so that a group's rows share work
 // (generate_groups_large, source/src/main.cu:50-247; backup/exp.cu:49-88).
 // Here rows with similar masks are put into the same 16-row chunk; the
-// product then sums the chunk's majority bits once (T = chunk_mask·B) and each
-// row only its difference from the chunk:
-//   C[r] = T + (row & ~mask)·B - (mask & ~row)·B.
+// product then sums a chunk base's bits once (T = base·B, the base being the
+// majority mask of a run of rows with the same leader; a chunk has up to two)
+// and each row only its difference from its base:
+//   C[r] = T + (row & ~base)·B - (base & ~row)·B.
 // When the rows of a chunk share most of their bits, that issues far fewer
 // vector adds than summing every row's own bits, and the vector adds are what
 // bounds the FP32 product on gfx950 (packed FP32 is not used).
@@ -15,8 +16,10 @@
 //     Hamming distance to the new leader and keeps the nearest; one 64-bit
 //     atomicMax per block of (distance, ~row) picks the next leader;
 //  2. rows stably sorted by nearest leader (hipCUB radix sort);
-//  3. chunk masks: a thread per (chunk, word) counts the chunk's bits;
-//  4. slot flags: a wave per slot compares popcount(row ^ mask) with
+//  3. chunk runs: a thread per chunk finds its two longest runs of equal
+//     leader; chunk masks: a thread per (chunk, base, word) counts the run's
+//     bits;
+//  4. slot flags: a wave per slot compares popcount(row ^ base_j) with
 //     popcount(row).
 #include <hipcub/hipcub.hpp>
 
@@ -100,26 +103,53 @@ k_cluster_iter(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
   }
 }
 
-// A thread per (chunk, word): majority bits of the chunk's rows.
+// A thread per chunk: the two longest runs of equal leader among the chunk's
+// slots (sorted keys), the earlier run first on equal lengths.
+__global__ void __launch_bounds__(256)
+k_chunk_runs(const int* __restrict__ keys, int64_t M, int4* __restrict__ runs) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= ceil_div(M, kChunk)) return;
+  const int n = (int)std::min<int64_t>(kChunk, M - kChunk * c);
+  const int* k = keys + kChunk * c;
+  int s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+  for (int i = 0; i < n;) {
+    int j = i + 1;
+    while (j < n && k[j] == k[i]) ++j;
+    const int len = j - i;
+    if (len > n0) {
+      s1 = s0, n1 = n0;
+      s0 = i, n0 = len;
+    } else if (len > n1) {
+      s1 = i, n1 = len;
+    }
+    i = j;
+  }
+  runs[c] = make_int4(s0, n0, s1, n1);
+}
+
+// A thread per (chunk, base, word): majority bits of the base's run.
 __global__ void __launch_bounds__(256)
 k_chunk_mask(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-             const int* __restrict__ perm, uint32_t* __restrict__ CM) {
+             const int* __restrict__ perm, const int4* __restrict__ runs,
+             uint32_t* __restrict__ CM) {
   const int64_t wpr = words_for(K);
   const int64_t chunks = ceil_div(M, kChunk);
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= chunks * ldw) return;
-  const int64_t c = t / ldw, w = t - c * ldw;
-  if (w >= wpr) {
+  if (t >= 2 * chunks * ldw) return;
+  const int64_t cb = t / ldw, w = t - cb * ldw;
+  const int64_t c = cb >> 1;
+  const int4 rn = runs[c];
+  const int start = (cb & 1) ? rn.z : rn.x, n = (cb & 1) ? rn.w : rn.y;
+  if (w >= wpr || n == 0) {
     CM[t] = 0u;
     return;
   }
-  const int n = (int)std::min<int64_t>(kChunk, M - kChunk * c);
   const uint32_t vb = valid_bits(w, K);
   uint32_t cnt[32];
 #pragma unroll
   for (int b = 0; b < 32; ++b) cnt[b] = 0;
   for (int i = 0; i < n; ++i) {
-    const uint32_t x = W[(int64_t)perm[kChunk * c + i] * ldw + w] & vb;
+    const uint32_t x = W[(int64_t)perm[kChunk * c + start + i] * ldw + w] & vb;
 #pragma unroll
     for (int b = 0; b < 32; ++b) cnt[b] += (x >> b) & 1u;
   }
@@ -129,8 +159,8 @@ k_chunk_mask(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   CM[t] = out;
 }
 
-// A wave per slot: the row, flagged when its difference from the chunk mask
-// has fewer bits than the row itself.
+// A wave per slot: the row, flagged with the base whose difference from the
+// row has the fewest bits when that is fewer than the row's own.
 __global__ void __launch_bounds__(256)
 k_slot_flags(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
              const int* __restrict__ perm, const uint32_t* __restrict__ CM,
@@ -141,16 +171,23 @@ k_slot_flags(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   const int64_t wpr = words_for(K);
   const int r = perm[s];
   const uint32_t* a = W + (int64_t)r * ldw;
-  const uint32_t* m = CM + (s / kChunk) * ldw;
-  int pa = 0, pd = 0;
+  const uint32_t* m0 = CM + (s / kChunk) * 2 * ldw;
+  const uint32_t* m1 = m0 + ldw;
+  int pa = 0, p0 = 0, p1 = 0;
   for (int64_t w = lane; w < wpr; w += 64) {
     const uint32_t x = a[w] & valid_bits(w, K);
     pa += __popc(x);
-    pd += __popc(x ^ m[w]);
+    p0 += __popc(x ^ m0[w]);
+    p1 += __popc(x ^ m1[w]);
   }
   pa = wave_sum(pa);
-  pd = wave_sum(pd);
-  if (lane == 0) slot_row[s] = r | (pd < pa ? BWS_SLOT_DELTA : 0);
+  p0 = wave_sum(p0);
+  p1 = wave_sum(p1);
+  int32_t flag = 0;
+  int best = pa;
+  if (p0 < best) flag = BWS_SLOT_DELTA, best = p0;
+  if (p1 < best) flag = BWS_SLOT_DELTA | BWS_SLOT_BASE1;
+  if (lane == 0) slot_row[s] = r | flag;
 }
 
 struct ClusterWs {
@@ -159,6 +196,7 @@ struct ClusterWs {
   int* iota;
   int* keys_out;
   int* perm;
+  int4* runs;
   unsigned long long* slots;
   void* sort_tmp;
   size_t sort_bytes;
@@ -177,6 +215,7 @@ ClusterWs carve_cluster(void* base, int64_t M, int L) {
   c.iota = cv.take<int>((size_t)M);
   c.keys_out = cv.take<int>((size_t)M);
   c.perm = cv.take<int>((size_t)M);
+  c.runs = cv.take<int4>((size_t)ceil_div(M, kChunk));
   c.slots = cv.take<unsigned long long>((size_t)L);
   c.sort_tmp = cv.take<uint8_t>(sort_bytes + 256);
   c.sort_bytes = sort_bytes + 256;
@@ -189,12 +228,13 @@ ClusterWs carve_cluster(void* base, int64_t M, int L) {
 // columns; B is staged 64 K rows at a time into LDS with
 // global_load_lds_dwordx4 (double-buffered, 2 x 64 KiB), as in the plain FP32
 // kernel.  In each wave lane l < 16 holds slot l's bitmap words (word 2s in
-// lanes 0..15, word 2s+1 in lanes 32..47) and lane 16 / 48 the chunk mask's.
-// Per stage every lane forms its add mask (row & ~mask for a delta row, the
-// row otherwise; lane 16: the mask itself when some row of the chunk is a
-// delta row) and its subtract mask (mask & ~row for a delta row), bit
-// reversed once in VALU; the rows then walk their bits with scalar bit scans,
-// each bit adding (or subtracting) one LDS row of 4 floats per lane.
+// lanes 0..15, word 2s+1 in lanes 32..47) and lanes 16 / 17 (48 / 49) the
+// chunk's two bases'.  Per stage every lane forms its add mask (row & ~base
+// for a delta row, the row otherwise; lanes 16, 17: the base itself when some
+// row of the chunk uses it) and its subtract mask (base & ~row for a delta
+// row), bit reversed once in VALU; the rows then walk their bits with scalar
+// bit scans, each bit adding (or subtracting) one LDS row of 4 floats per
+// lane.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void take_bit(uint64_t& m, int& k) {
   asm volatile("s_ff1_i32_b64 %0, %1" : "=s"(k) : "s"(m));
@@ -275,29 +315,34 @@ k_spmm_cluster(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
   const int64_t wpr = (K + 31) / 32;
   const int rl = lane & 31;
 
-  // slot rows: lane l < 16 (and l + 32) holds slot l's row and delta flag
+  // slot rows: lane l < 16 (and l + 32) holds slot l's row and flags
   int row = -1;
-  bool delta = false;
+  bool delta = false, base1 = false;
   if (rl < kChunk && slot0 + rl < M) {
     const int32_t v = slot_row[slot0 + rl];
-    row = v & (BWS_SLOT_DELTA - 1);
+    row = v & (BWS_SLOT_BASE1 - 1);
     delta = (v & BWS_SLOT_DELTA) != 0;
+    base1 = (v & BWS_SLOT_BASE1) != 0;
   }
-  const uint64_t dmask = __ballot(delta && lane < kChunk);  // slots summed as deltas
+  // slots summed as differences from base 0 / base 1
+  const uint64_t dmask0 = __ballot(delta && !base1 && lane < kChunk);
+  const uint64_t dmask1 = __ballot(delta && base1 && lane < kChunk);
   const uint32_t* src = nullptr;
   if (rl < kChunk) {
     if (row >= 0) src = W + (int64_t)row * ldw;
-  } else if (rl == kChunk && dmask != 0ull) {
-    src = chunk_mask + chunk * ldw;
+  } else if (rl == kChunk && dmask0 != 0ull) {
+    src = chunk_mask + 2 * chunk * ldw;
+  } else if (rl == kChunk + 1 && dmask1 != 0ull) {
+    src = chunk_mask + (2 * chunk + 1) * ldw;
   }
   const int64_t ncol = (n0 + 4 * lane + 4 <= N) ? n0 + 4 * lane : N - 4;  // clamped column
   const float* bcol = B + ncol;
-  const uint32_t base0 = lds_u32(&sB0[0][0]), base1 = lds_u32(&sB1[0][0]);
+  const uint32_t lds0 = lds_u32(&sB0[0][0]), lds1 = lds_u32(&sB1[0][0]);
 
   float4 acc[kChunk];
 #pragma unroll
   for (int r = 0; r < kChunk; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 accT = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 accT0 = make_float4(0.f, 0.f, 0.f, 0.f), accT1 = accT0;
 
   auto issue = [&](int64_t st, uint32_t base) {
 #pragma unroll
@@ -317,7 +362,7 @@ k_spmm_cluster(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
     return kvalid >= 64 ? ~0ull : (~0ull << (64 - kvalid));
   };
 
-  issue(0, base0);
+  issue(0, lds0);
   uint32_t wcur = load_words(0);
   uint32_t wnext = load_words(1);
 
@@ -333,9 +378,11 @@ k_spmm_cluster(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
 
     const uint32_t wo = (uint32_t)__shfl_xor((int)wcur, 32, 64);
     const uint64_t mm = ((((uint64_t)wcur) << 32) | wo) & smask;
-    const uint64_t bm = readlane_u64((uint32_t)(mm >> 32), (uint32_t)mm, kChunk);
+    const uint64_t bm0 = readlane_u64((uint32_t)(mm >> 32), (uint32_t)mm, kChunk);
+    const uint64_t bm1 = readlane_u64((uint32_t)(mm >> 32), (uint32_t)mm, kChunk + 1);
     uint64_t am = mm, sm = 0ull;
     if (delta && lane < kChunk) {
+      const uint64_t bm = base1 ? bm1 : bm0;
       am = mm & ~bm;
       sm = bm & ~mm;
     }
@@ -353,15 +400,19 @@ k_spmm_cluster(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
         walk_bits<true>(acc[r], readlane_u64(s_hi, s_lo, r), __builtin_amdgcn_readlane(s_n, r),
                         cur, lane);
     }
-    walk_bits<false>(accT, readlane_u64(a_hi, a_lo, kChunk),
-                     __builtin_amdgcn_readlane(a_n, kChunk), cur, lane);
+    if (dmask0)
+      walk_bits<false>(accT0, readlane_u64(a_hi, a_lo, kChunk),
+                       __builtin_amdgcn_readlane(a_n, kChunk), cur, lane);
+    if (dmask1)
+      walk_bits<false>(accT1, readlane_u64(a_hi, a_lo, kChunk + 1),
+                       __builtin_amdgcn_readlane(a_n, kChunk + 1), cur, lane);
     wcur = wnext;
     wnext = wn2;
   };
 
   for (int64_t st = 0; st < nst; st += 2) {
-    stage(st, sB0, base1);
-    if (st + 1 < nst) stage(st + 1, sB1, base0);
+    stage(st, sB0, lds1);
+    if (st + 1 < nst) stage(st + 1, sB1, lds0);
   }
 
   const int64_t n = n0 + 4 * lane;
@@ -371,7 +422,8 @@ k_spmm_cluster(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
     const int rr = __builtin_amdgcn_readlane(row, r);
     if (rr < 0) continue;
     float4 v = acc[r];
-    if ((dmask >> r) & 1ull) acc4<false>(v, accT);
+    if ((dmask0 >> r) & 1ull) acc4<false>(v, accT0);
+    if ((dmask1 >> r) & 1ull) acc4<false>(v, accT1);
     float4* c = reinterpret_cast<float4*>(C + (int64_t)rr * ldc + n);
     v.x *= alpha;
     v.y *= alpha;
@@ -396,7 +448,7 @@ using namespace bws;
 extern "C" {
 
 size_t bws_cluster_workspace_size(int64_t rows, int32_t max_clusters) {
-  if (rows <= 0 || rows >= BWS_SLOT_DELTA || max_clusters < 0) return 0;
+  if (rows <= 0 || rows >= BWS_SLOT_BASE1 || max_clusters < 0) return 0;
   return carve_cluster(nullptr, rows, leaders_for(rows, max_clusters)).total + 256;
 }
 
@@ -406,7 +458,7 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
   BWS_TRY(check_bitmap(A, false));
   if (A->values) return BWS_EUNSUPPORTED;
   if (max_clusters < 0) return BWS_EINVAL;
-  if (A->rows >= BWS_SLOT_DELTA) return BWS_ECAPACITY;
+  if (A->rows >= BWS_SLOT_BASE1) return BWS_ECAPACITY;
   if (A->rows == 0) return BWS_OK;
   if (!slot_row || !chunk_mask) return BWS_EINVAL;
   const int64_t M = A->rows;
@@ -430,9 +482,13 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
                                          (int)M, 0, key_bits(L), s) != hipSuccess)
     return BWS_EHIP;
   BWS_TRY(launch_check());
-  const int64_t threads = ceil_div(M, kChunk) * A->ldw;
+  const int64_t chunks = ceil_div(M, kChunk);
+  hipLaunchKernelGGL(k_chunk_runs, dim3((unsigned)ceil_div(chunks, 256)), dim3(256), 0, s,
+                     c.keys_out, M, c.runs);
+  BWS_TRY(launch_check());
+  const int64_t threads = 2 * chunks * A->ldw;
   hipLaunchKernelGGL(k_chunk_mask, dim3((unsigned)ceil_div(threads, 256)), dim3(256), 0, s,
-                     A->words, M, A->cols, A->ldw, c.perm, chunk_mask);
+                     A->words, M, A->cols, A->ldw, c.perm, c.runs, chunk_mask);
   BWS_TRY(launch_check());
   hipLaunchKernelGGL(k_slot_flags, dim3((unsigned)ceil_div(M, 4)), dim3(256), 0, s, A->words, M,
                      A->cols, A->ldw, c.perm, chunk_mask, slot_row);
@@ -445,7 +501,7 @@ bws_status bws_spmm_clustered(const bws_bitmap* A, const int32_t* slot_row,
   BWS_TRY(check_bitmap(A, false));
   if (A->values) return BWS_EUNSUPPORTED;
   if (N < 0 || !ld_ok(N, ldc) || (A->cols > 0 && !ld_ok(N, ldb))) return BWS_EINVAL;
-  if (A->rows >= BWS_SLOT_DELTA) return BWS_ECAPACITY;
+  if (A->rows >= BWS_SLOT_BASE1) return BWS_ECAPACITY;
   if (A->rows == 0 || N == 0) return BWS_OK;
   if (!C || (A->cols > 0 && !B)) return BWS_EINVAL;
   hipStream_t s = stream;

@@ -262,21 +262,26 @@ bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64
  * (generate_groups_large, source/src/main.cu:50-247; the row-to-group intent
  * of backup/exp.cu:49-88); this analysis step has no reference counterpart:
  * it orders the rows so that each run of 16 slots holds similar masks, and
- * bws_spmm_clustered sums a chunk's shared bits once.
+ * bws_spmm_clustered sums each chunk base's bits once.
  *   leaders: farthest point over Hamming distance.  Leader 0 is row 0; every
  *     row keeps its nearest leader (the earliest on ties); leader i+1 is the
  *     row farthest from its nearest leader (the lowest row on ties);
  *   order:   rows by nearest leader, ascending within a leader;
- *   chunks:  chunk c is slots 16c .. 16c+15 (the last may hold fewer);
- *            chunk_mask[c] has bit k set iff 2 * (rows of the chunk with bit
- *            k) > (rows in the chunk) + 1;
- *   slots:   slot_row[s] = row of slot s, plus BWS_SLOT_DELTA when
- *            popcount(row ^ chunk_mask) < popcount(row).
+ *   chunks:  chunk c is slots 16c .. 16c+15 (the last may hold fewer); its
+ *            slots form runs of equal leader.  Base 0 comes from the longest
+ *            run, base 1 from the second longest (the earlier run on equal
+ *            lengths; none when the chunk has one run): bit k of base j is set
+ *            iff 2 * (rows of the run with bit k) > (rows in the run) + 1.
+ *            chunk_mask[(2c + j) * A->ldw + w] holds base j;
+ *   slots:   slot_row[s] = row of slot s, plus BWS_SLOT_DELTA when some base
+ *            has popcount(row ^ base) < popcount(row) (the smallest wins, base
+ *            0 on ties), plus BWS_SLOT_BASE1 when that base is base 1.
  * max_clusters = 0 uses rows / 64 leaders; the count is clamped to
  * [1, min(rows, 65536)].  slot_row has `rows` entries, chunk_mask
- * ceil(rows/16) x A->ldw words.  0/1 masks only (A->values must be NULL);
- * rows < 2^30.  Deterministic: the same bitmap gives the same plan. */
+ * ceil(rows/16) x 2 x A->ldw words.  0/1 masks only (A->values must be
+ * NULL); rows < 2^29.  Deterministic: the same bitmap gives the same plan. */
 #define BWS_SLOT_DELTA (1 << 30)
+#define BWS_SLOT_BASE1 (1 << 29)
 size_t bws_cluster_workspace_size(int64_t rows, int32_t max_clusters);
 bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* slot_row,
                             uint32_t* chunk_mask, void* workspace, size_t workspace_bytes,
@@ -348,13 +353,12 @@ bws_status bws_spmm_groups(const bws_bitmap* A, const int32_t* group_id,
 
 /* C = alpha * (A · B) + beta * C for a 0/1 bitmap A, FP32 B and C, with the
  * plan of bws_cluster_rows for this A: a wave takes one chunk; each delta row
- * is accumulated as (row & ~chunk_mask)·B - (chunk_mask & ~row)·B and the
- * chunk's chunk_mask·B, summed once, is added at the end; other rows sum
- * their own bits.  FP32 vector adds; the summation order differs from
- * bws_spmm's, and the result stays within FP32 rounding of A·B relative to
- * |A|·|B| (the subtracted terms are bits of the chunk mask).  Needs N, ldb and
- * ldc multiples of 4 and 16-byte aligned B and C (BWS_EUNSUPPORTED
- * otherwise). */
+ * is accumulated as (row & ~base)·B - (base & ~row)·B and its base's base·B,
+ * summed once per chunk, is added at the end; other rows sum their own bits.
+ * FP32 vector adds; the summation order differs from bws_spmm's, and the
+ * result stays within FP32 rounding of A·B relative to |A|·|B| (the
+ * subtracted terms are bits of the base).  Needs N, ldb and ldc multiples of
+ * 4 and 16-byte aligned B and C (BWS_EUNSUPPORTED otherwise). */
 bws_status bws_spmm_clustered(const bws_bitmap* A, const int32_t* slot_row,
                               const uint32_t* chunk_mask, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, float alpha, float beta, bws_stream stream);

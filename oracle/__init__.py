@@ -223,11 +223,12 @@ def cluster_leaders(rows, max_clusters=0):
 
 
 def cluster_rows(W, cols, max_clusters=0):
-    """Row-clustering plan: (slot_row int32 [rows], chunk_mask uint32 [ceil(rows/16), ldw])."""
+    """Row-clustering plan: (slot_row int32 [rows], chunk_mask uint32
+    [ceil(rows/16), 2, ldw] (two bases per chunk))."""
     W = _c(W, np.uint32)
     rows, ldw = W.shape
     slot_row = np.empty(rows, np.int32)
-    chunk_mask = np.empty(((rows + 15) // 16, ldw), np.uint32)
+    chunk_mask = np.empty(((rows + 15) // 16, 2, ldw), np.uint32)
     if rows:
         lib().oref_cluster_rows(_p(W), rows, cols, ldw, max_clusters, _p(slot_row),
                                 _p(chunk_mask))

@@ -354,10 +354,15 @@ int32_t oref_cluster_leaders(int64_t rows, int32_t max_clusters) {
  *      leader on ties), leader i+1 is the row with the largest distance to
  *      its nearest leader (lowest row on ties).
  *   2. Rows ordered by nearest leader, ascending row within a leader.
- *   3. Chunk c = slots 16c .. 16c+15 (fewer in the last); chunk_mask bit k is
- *      set iff 2 * (the chunk's rows with bit k) > (rows in the chunk) + 1.
- *   4. slot_row[s] = row | (1 << 30) when popcount(row ^ chunk_mask) <
- *      popcount(row) (the row is then summed as a delta from the chunk). */
+ *   3. Chunk c = slots 16c .. 16c+15 (fewer in the last).  Its rows form runs
+ *      of equal leader; base 0 is made from the longest run, base 1 from the
+ *      second longest (earlier run first on equal lengths; no base 1 when the
+ *      chunk has one run): bit k of base j is set iff 2 * (rows of run j with
+ *      bit k) > (rows in run j) + 1.  chunk_mask[(2c + j) * ldw + w].
+ *   4. slot_row[s] = row, plus (1 << 30) when the row is summed as a
+ *      difference from a base: the base j minimising popcount(row ^ base_j),
+ *      if that is below popcount(row) (base 0 on ties); (1 << 29) marks
+ *      base 1. */
 void oref_cluster_rows(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
                        int32_t max_clusters, int32_t* slot_row, uint32_t* chunk_mask) {
   if (rows <= 0) return;
@@ -390,28 +395,51 @@ void oref_cluster_rows(const uint32_t* W, int64_t rows, int64_t cols, int64_t ld
   const int64_t chunks = (rows + 15) / 16;
   for (int64_t c = 0; c < chunks; ++c) {
     const int64_t n = rows - 16 * c < 16 ? rows - 16 * c : 16;
-    for (int64_t w = 0; w < ldw; ++w) {
-      uint32_t m = 0u;
-      if (w < wpr) {
-        const uint32_t vb = valid_word(w, cols);
-        for (int b = 0; b < 32; ++b) {
-          int64_t cnt = 0;
-          for (int64_t i = 0; i < n; ++i) cnt += (W[(int64_t)perm[16 * c + i] * ldw + w] & vb & WBIT(b)) != 0;
-          if (2 * cnt > n + 1) m |= WBIT(b);
-        }
+    const int32_t* pr = perm + 16 * c;
+    /* the two longest runs of equal leader */
+    int64_t s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+    for (int64_t i = 0; i < n;) {
+      int64_t j = i + 1;
+      while (j < n && owner[pr[j]] == owner[pr[i]]) ++j;
+      const int64_t len = j - i;
+      if (len > n0) {
+        s1 = s0, n1 = n0;
+        s0 = i, n0 = len;
+      } else if (len > n1) {
+        s1 = i, n1 = len;
       }
-      chunk_mask[c * ldw + w] = m;
+      i = j;
+    }
+    for (int b = 0; b < 2; ++b) {
+      const int64_t bs = b == 0 ? s0 : s1, bn = b == 0 ? n0 : n1;
+      for (int64_t w = 0; w < ldw; ++w) {
+        uint32_t m = 0u;
+        if (w < wpr && bn > 0) {
+          const uint32_t vb = valid_word(w, cols);
+          for (int k = 0; k < 32; ++k) {
+            int64_t cnt = 0;
+            for (int64_t i = 0; i < bn; ++i) cnt += (W[(int64_t)pr[bs + i] * ldw + w] & vb & WBIT(k)) != 0;
+            if (2 * cnt > bn + 1) m |= WBIT(k);
+          }
+        }
+        chunk_mask[(2 * c + b) * ldw + w] = m;
+      }
     }
   }
   for (int64_t s = 0; s < rows; ++s) {
     const int64_t r = perm[s], c = s / 16;
-    int64_t pa = 0, pd = 0;
+    int64_t pa = 0, p0 = 0, p1 = 0;
     for (int64_t w = 0; w < wpr; ++w) {
       const uint32_t a = W[r * ldw + w] & valid_word(w, cols);
       pa += popc32(a);
-      pd += popc32(a ^ chunk_mask[c * ldw + w]);
+      p0 += popc32(a ^ chunk_mask[(2 * c) * ldw + w]);
+      p1 += popc32(a ^ chunk_mask[(2 * c + 1) * ldw + w]);
     }
-    slot_row[s] = (int32_t)r | (pd < pa ? (int32_t)(1 << 30) : 0);
+    int32_t flag = 0;
+    int64_t best = pa;
+    if (p0 < best) flag = 1 << 30, best = p0;
+    if (p1 < best) flag = (1 << 30) | (1 << 29);
+    slot_row[s] = (int32_t)r | flag;
   }
   free(dist);
   free(owner);

@@ -2,19 +2,22 @@
 
 The plan has no reference counterpart (it is this library's analysis step for
 the FP32 product), so it is pinned by what fixes it:
-1. a hand-worked case whose leaders, order and masks follow from the rules;
+1. hand-worked cases whose leaders, order, bases and flags follow from the
+   rules;
 2. exact clusters: rows that are copies of a few patterns must come out in
-   chunks of identical rows, each chunk mask equal to its pattern and every
-   row a zero-difference delta row;
+   chunks of identical rows, base 0 equal to the pattern, no base 1, and
+   every row a zero-difference delta row;
 3. outputs checked from first principles on random masks: slot_row is a
-   permutation, each chunk mask is its chunk's majority, each flag follows
-   the popcount rule, and the leader order is farthest-point;
-4. the algebra the clustered product relies on: T + (row & ~mask)·B -
-   (mask & ~row)·B equals row·B, evaluated in FP64 against the FP64 oracle."""
+   permutation in farthest-point leader order, each base is the majority of
+   its run, each flag follows the popcount rule;
+4. the algebra the clustered product relies on: T + (row & ~base)·B -
+   (base & ~row)·B equals row·B, evaluated in FP64 against the FP64 oracle."""
 import numpy as np
 import pytest
 
 DELTA = 1 << 30
+BASE1 = 1 << 29
+ROW = BASE1 - 1
 
 
 def words(bits):
@@ -45,26 +48,45 @@ def test_hand_worked_order(oracle):
     # A = 0xFFFF0000, A' = A | col 31, B = 0x0000FFFF, B' = B & ~col 31, rows
     # [A, B', A', B]: leader 0 = row 0; distances 0, 31, 1, 32 -> leader 1 =
     # row 3; row 1 is 1 from B (31 from A), so owners [0, 1, 0, 1] and the
-    # order is rows 0, 2, 1, 3.  One chunk of 4 rows: no column is set in 3 of
-    # them, so the mask is empty and no row is a delta row.
+    # order is rows 0, 2, 1, 3.  The chunk has runs {0, 2} and {1, 3} of two
+    # rows: base 0 = bits in both of A, A' = A, base 1 = B' (B & B'); every
+    # row is within one bit of its run's base.
     W = np.array([[0xFFFF0000], [0x0000FFFE], [0xFFFF0001], [0x0000FFFF]], np.uint32)
     slot, cm = oracle.cluster_rows(W, 32, max_clusters=2)
-    assert slot.tolist() == [0, 2, 1, 3]
-    assert cm.tolist() == [[0]]
-    # one leader: every row belongs to row 0, order is ascending rows
-    slot, _ = oracle.cluster_rows(W, 32, max_clusters=1)
+    assert slot.tolist() == [0 | DELTA, 2 | DELTA, 1 | DELTA | BASE1, 3 | DELTA | BASE1]
+    assert cm.tolist() == [[[0xFFFF0000], [0x0000FFFE]]]
+    # one leader: one run of four rows, no column set in three of them: no
+    # base, no delta row, ascending order
+    slot, cm = oracle.cluster_rows(W, 32, max_clusters=1)
     assert slot.tolist() == [0, 1, 2, 3]
+    assert cm.tolist() == [[[0], [0]]]
 
 
 def test_hand_worked_majority(oracle):
-    # 3 copies of P and one empty row in one chunk (n = 4): bits of P are in 3
-    # rows, 2*3 > 5, so the mask is P; P rows are delta rows (0 < popcount P),
-    # the empty row is not (popcount(P) > 0).
+    # 3 copies of P and one empty row in one run (n = 4): bits of P are in 3
+    # rows, 2*3 > 5, so base 0 is P; P rows are delta rows (0 < popcount P),
+    # the empty row is not (popcount(P) > 0 and the empty base 1 ties).
     P = 0x0F0F00F1
     W = np.array([[P], [0], [P], [P]], np.uint32)
     slot, cm = oracle.cluster_rows(W, 32, max_clusters=1)
-    assert cm.tolist() == [[P]]
+    assert cm.tolist() == [[[P], [0]]]
     assert slot.tolist() == [0 | DELTA, 1, 2 | DELTA, 3 | DELTA]
+
+
+def test_hand_worked_runs(oracle):
+    # 16 rows in one chunk from three patterns: X x7 (rows 0..6), Y x5, Z x4.
+    # Leaders: row 0 (X), then the farthest pattern, then the last; runs in
+    # leader order have lengths 7 (X), and 5 / 4: base 0 = X, base 1 = Y;
+    # Z rows are direct (Z is far from X and Y).
+    X, Y, Z = 0xFF000000, 0x00FF0000, 0x0000FFFF
+    W = np.array([[X]] * 7 + [[Y]] * 5 + [[Z]] * 4, np.uint32)
+    slot, cm = oracle.cluster_rows(W, 32, max_clusters=3)
+    rows = slot & ROW
+    assert sorted(rows.tolist()) == list(range(16))
+    assert cm[0, 0, 0] == X and cm[0, 1, 0] == Y
+    for s_, r in zip(slot, rows):
+        want = {X: DELTA, Y: DELTA | BASE1, Z: 0}[int(W[r, 0])]
+        assert s_ & (DELTA | BASE1) == want
 
 
 @pytest.mark.parametrize("patterns,reps,cols", [(4, 32, 200), (7, 16, 96), (3, 48, 1000)])
@@ -73,13 +95,14 @@ def test_exact_clusters(oracle, rng, patterns, reps, cols):
     idx = rng.permutation(np.repeat(np.arange(patterns), reps))
     W = words(P[idx])
     slot, cm = oracle.cluster_rows(W, cols, max_clusters=patterns)
-    rows = slot & (DELTA - 1)
+    rows = slot & ROW
     assert sorted(rows.tolist()) == list(range(len(idx)))
-    assert ((slot & DELTA) != 0).all()
+    assert ((slot & (DELTA | BASE1)) == DELTA).all()
     for c in range(len(idx) // 16):
         pats = idx[rows[16 * c:16 * c + 16]]
         assert len(set(pats.tolist())) == 1, "a chunk mixes clusters"
-        assert (cm[c] == words(P[pats[:1]])[0]).all()
+        assert (cm[c, 0] == words(P[pats[:1]])[0]).all()
+        assert (cm[c, 1] == 0).all()
 
 
 @pytest.mark.parametrize("rows,cols,L,p", [(37, 45, 0, 0.2), (100, 64, 5, 0.1), (333, 129, 0, 0.05),
@@ -89,7 +112,7 @@ def test_first_principles(oracle, rng, rows, cols, L, p):
     b[rng.random(rows) < 0.1] = False  # some empty rows
     W = words(b)
     slot, cm = oracle.cluster_rows(W, cols, max_clusters=L)
-    order = slot & (DELTA - 1)
+    order = slot & ROW
     assert sorted(order.tolist()) == list(range(rows))
     nl = oracle.cluster_leaders(rows, L)
     # farthest-point leaders and nearest-leader owners, restated from the rules
@@ -104,17 +127,32 @@ def test_first_principles(oracle, rng, rows, cols, L, p):
         lead = int(np.argmax(dist))
     expect = sorted(range(rows), key=lambda r: (owner[r], r))
     assert order.tolist() == expect
+    valid = words(np.ones((1, cols), bool))[0]
     for c in range((rows + 15) // 16):
         rr = order[16 * c:16 * c + 16]
-        cnt = b[rr].sum(0)
-        maj = 2 * cnt > len(rr) + 1
-        assert (bits_of(cm[c:c + 1], cols)[0] == maj).all()
-        assert (cm[c] & ~words(np.ones((1, cols), bool))[0] == 0).all()
+        runs = []  # (length, start) of runs of equal owner
+        i = 0
+        while i < len(rr):
+            j = i
+            while j < len(rr) and owner[rr[j]] == owner[rr[i]]:
+                j += 1
+            runs.append((j - i, i))
+            i = j
+        runs.sort(key=lambda t: (-t[0], t[1]))
+        for j in range(2):
+            if j < len(runs):
+                n, st = runs[j]
+                maj = 2 * b[rr[st:st + n]].sum(0) > n + 1
+            else:
+                maj = np.zeros(cols, bool)
+            assert (bits_of(cm[c, j:j + 1], cols)[0] == maj).all()
+            assert (cm[c, j] & ~valid == 0).all()
     for s in range(rows):
         r = order[s]
-        mask = bits_of(cm[s // 16:s // 16 + 1], cols)[0]
-        is_delta = (b[r] ^ mask).sum() < b[r].sum()
-        assert bool(slot[s] & DELTA) == bool(is_delta)
+        base = [bits_of(cm[s // 16, j:j + 1], cols)[0] for j in range(2)]
+        costs = [int(b[r].sum())] + [int((b[r] ^ m).sum()) for m in base]
+        best = min(range(3), key=lambda i: (costs[i], i))
+        assert slot[s] & (DELTA | BASE1) == [0, DELTA, DELTA | BASE1][best]
 
 
 def test_delta_algebra_fp64(oracle, bws):
@@ -125,13 +163,15 @@ def test_delta_algebra_fp64(oracle, bws):
     assert ((slot & DELTA) != 0).mean() > 0.8  # the clusters are found
     R = oracle.spmm_f64(W, K, B)
     Cp = np.zeros((M, N))
-    T = oracle.spmm_f64(cm, K, B)  # chunk sums, [chunks, N]
+    T = oracle.spmm_f64(cm.reshape(-1, cm.shape[2]), K, B).reshape(-1, 2, N)  # base sums
     for s in range(M):
-        r = slot[s] & (DELTA - 1)
+        r = slot[s] & ROW
         if slot[s] & DELTA:
-            add = W[r] & ~cm[s // 16]
-            sub = cm[s // 16] & ~W[r]
-            Cp[r] = (T[s // 16] + oracle.spmm_f64(add[None], K, B)[0]
+            j = 1 if slot[s] & BASE1 else 0
+            base = cm[s // 16, j]
+            add = W[r] & ~base
+            sub = base & ~W[r]
+            Cp[r] = (T[s // 16, j] + oracle.spmm_f64(add[None], K, B)[0]
                      - oracle.spmm_f64(sub[None], K, B)[0])
         else:
             Cp[r] = oracle.spmm_f64(W[r:r + 1], K, B)[0]
