@@ -3,11 +3,15 @@
 
 Workload (default): C = A·B, M = N = K = 4096, A a clustered 0/1 bitmap
 (64 prototypes ~ Bernoulli(0.03125) with 2% bit flips, rows shuffled: ~5%
-dense, i.e. "95% sparse"), B ~ N(0,1) FP32, FP32 path.  A step is one
-bws_spmm launch over the resident bitmap and B (the reference's timed region
-is likewise the GEMM on pre-packed operands).  Prints ONE JSON line with
-effective TFLOP/s = 2·nnz·N / t, the HBM roofline of the kernel, and the
-reference CPU naive loop timed on a bounded row sample of the same workload.
+dense, i.e. "95% sparse"), B ~ N(0,1) FP32, FP32 path.  A step is one SpMM
+launch over the resident bitmap and B (the reference's timed region is
+likewise the GEMM on pre-packed, pre-grouped operands).  For FP32 the step is
+the row-clustered product (bws_spmm_clustered) when the row-clustering plan,
+built once per bitmap and timed separately, sums at least half of the rows
+as differences from a chunk base; otherwise the plain bws_spmm.  Both
+kernels' times are reported.  Prints ONE JSON line with effective TFLOP/s =
+2·nnz·N / t, the HBM roofline of the kernel, and the reference CPU naive
+loop timed on a bounded row sample of the same workload.
 """
 import argparse
 import json
@@ -21,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFS = 2500.0  # dense BF16 / FP16
 # HBM bytes per launch of the benchmark kernel, from the FETCH_SIZE / WRITE_SIZE
-# counter runs summarised in profiles/ (one file per precision)
+# counter runs summarised in profiles/ (one file per kernel)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "spmm_{}_4096_clustered_traffic.json")
 
 
@@ -109,6 +113,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--quick", action="store_true", help="extras: skip the C4/C5/comparator legs")
+    ap.add_argument("--fp32-path", default="auto", choices=["auto", "clustered", "plain"],
+                    help="FP32 kernel: the row-clustered product, the plain one, or by the plan")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -148,23 +154,44 @@ def main():
     C = torch.empty((M, N), dtype=torch.float32, device=dev)
     log(f"workload M=N=K={M}x{N}x{K} nnz={nnz} density={nnz / (M * K):.4f} prec={args.precision}")
 
-    step = lambda: bws.spmm(A, Bin, args.precision, out=C)
+    plain = lambda: bws.spmm(A, Bin, args.precision, out=C)
+    step, path, plan_info, clusters = plain, args.precision, None, None
+    if args.precision == "fp32" and args.fp32_path != "plain":
+        # the row-clustering plan: an analysis step per bitmap, timed apart
+        clusters = bws.cluster_rows(A)
+        plan_ms = time_loop(lambda: bws.cluster_rows(A), 3, 1)[1]
+        delta = float(((clusters.slot_row & bws.SLOT_DELTA) != 0).float().mean().item())
+        plan_info = {"plan_ms": round(plan_ms, 4), "delta_row_fraction": round(delta, 4)}
+        if args.fp32_path == "clustered" or delta >= 0.5:
+            step = lambda: bws.spmm_clustered(A, clusters, Bin, out=C)
+            path = "fp32_rowclusters"
     spin_up(step)
     ms_step, ms_kernel = time_loop(step, args.steps, args.warmup)
     flops = 2.0 * nnz * N
     value = flops / (ms_step * 1e-3) / 1e12
+    if plan_info is not None:
+        # the other FP32 kernel on the same operands, for comparison
+        other = plain if path != "fp32" else (lambda: bws.spmm_clustered(A, clusters, Bin, out=C))
+        spin_up(other, 0.1)
+        o_ms = time_loop(other, max(args.steps // 2, 5), 2)[1]
+        plan_info["plain_ms" if path != "fp32" else "clustered_ms"] = round(o_ms, 5)
+        if path != "fp32" and o_ms > ms_kernel:
+            plan_info["plan_break_even_calls"] = round(plan_info["plan_ms"] / (o_ms - ms_kernel), 1)
     s_b = {"fp32": 4, "bf16x2": 4, "bf16": 2, "fp16": 2}[args.precision]
     alg_bytes = 4 * M * ((K + 31) // 32) + s_b * K * N + 4 * M * N
+    if path == "fp32_rowclusters":  # + slot rows and chunk bases
+        alg_bytes += 4 * M + 4 * 2 * ((M + 15) // 16) * ((K + 31) // 32)
     achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
     traffic = None
-    tfile = PROFILE_TRAFFIC.format(args.precision)
+    tfile = PROFILE_TRAFFIC.format(path)
     if os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125:
         try:
             traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     if args.precision == "fp32":
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        roofline = {"kernel": "k_spmm_cluster" if path == "fp32_rowclusters" else "k_spmm_rows2",
+                    "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel_ms": round(ms_kernel, 5), "algorithmic_bytes": alg_bytes}
     else:
@@ -181,7 +208,7 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        extras = run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args)
+        extras = run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters)
 
     cpu = None
     if not args.no_cpu:
@@ -209,6 +236,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if plan_info is not None:
+        out["config"]["fp32_path"] = "row clusters" if path == "fp32_rowclusters" else "plain"
+        out["fp32_paths"] = plan_info
     if extras:
         out["extras"] = extras
     print(json.dumps(out), flush=True)
@@ -221,7 +251,7 @@ def _err_s(C, R, S):
     return float(e.max()), float(e.mean())
 
 
-def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args):
+def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     """Pipeline stages, the precision variants with their error, forward +
     backward FP32 vs BF16, the value-carrying mask workload, the grouped batch
     in one captured hipGraph, and library comparators (dense GEMM, CSR SpMM).
@@ -267,6 +297,13 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args):
         f32 = tl(lambda: fb("fp32", Bd, dC), 10, 2)
         b16 = tl(lambda: fb("bf16", Bb, dCb), 10, 2)
         ex["fwd_bwd_01mask"] = {"fp32_ms": f32, "bf16_ms": b16, "speedup": round(f32 / b16, 3)}
+        if clusters is not None:
+            # FP32 with the row-clustered forward (the backward's Aᵀ is not
+            # clustered by this plan)
+            f32c = tl(lambda: (bws.spmm_clustered(A, clusters, Bd, out=C),
+                               bws.spmm_tA(AT, dC, "fp32", out=dB)), 10, 2)
+            ex["fwd_bwd_01mask"].update({"fp32_clustered_fwd_ms": f32c,
+                                         "speedup_vs_clustered_fwd": round(f32c / b16, 3)})
     except Exception as e:  # reported, never hidden
         ex["fwd_bwd_01mask"] = {"error": str(e)}
     del AT, dC, dB, R, S
