@@ -156,7 +156,7 @@ bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t
   if (precision == BWS_PREC_BF16X2 && b_dtype != BWS_DTYPE_F32) return BWS_EUNSUPPORTED;
   if (precision == BWS_PREC_BF16 && b_dtype == BWS_DTYPE_F16) return BWS_EUNSUPPORTED;
   if (precision == BWS_PREC_FP16 && b_dtype == BWS_DTYPE_BF16) return BWS_EUNSUPPORTED;
-  if (!workspace || workspace_bytes < bws_batch_workspace_size(count)) return BWS_EWORKSPACE;
+  if (!ws_ok(workspace, workspace_bytes, bws_batch_workspace_size(count))) return BWS_EWORKSPACE;
   memset(batch, 0, sizeof(*batch));
   batch->count = count;
   batch->b_dtype = b_dtype;

@@ -39,6 +39,11 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 inline bool dims_ok(int64_t rows, int64_t cols) { return rows >= 0 && cols >= 0; }
 inline bool ldw_ok(int64_t cols, int64_t ldw) { return ldw >= words_for(cols) && ldw >= 1; }
 inline bool ld_ok(int64_t cols, int64_t ld) { return ld >= cols && ld >= 1; }
+// Caller workspaces: present, large enough and 256-byte aligned (the
+// alignment hipMalloc gives; sub-buffers are carved at 256-byte offsets).
+inline bool ws_ok(const void* ws, size_t have, size_t need) {
+  return ws && have >= need && (reinterpret_cast<uintptr_t>(ws) & 255u) == 0;
+}
 bws_status check_bitmap(const bws_bitmap* A, bool need_row_ptr);
 
 // Maximum grid dimension used for 1-D launches (blocks are looped over).

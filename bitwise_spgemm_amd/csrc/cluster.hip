@@ -463,7 +463,7 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
   if (!slot_row || !chunk_mask) return BWS_EINVAL;
   const int64_t M = A->rows;
   const int L = leaders_for(M, max_clusters);
-  if (!workspace || workspace_bytes < bws_cluster_workspace_size(M, max_clusters))
+  if (!ws_ok(workspace, workspace_bytes, bws_cluster_workspace_size(M, max_clusters)))
     return BWS_EWORKSPACE;
   hipStream_t s = stream;
   ClusterWs c = carve_cluster(workspace, M, L);

@@ -181,7 +181,8 @@ bws_status bws_coo_to_bitmap(const int32_t* row_idx, const int32_t* col_idx, con
   if (nnz > 0 && (!row_idx || !col_idx)) return BWS_EINVAL;
   if (values_out && nnz > 0 && !values) return BWS_EINVAL;
   if (rows > 0 && !W) return BWS_EINVAL;
-  if (!workspace || workspace_bytes < bws_convert_workspace_size(rows, cols)) return BWS_EWORKSPACE;
+  if (!ws_ok(workspace, workspace_bytes, bws_convert_workspace_size(rows, cols)))
+    return BWS_EWORKSPACE;
   hipStream_t s = stream;
   const ConvWs c = carve_conv(workspace, rows, cols);
   hipLaunchKernelGGL(k_clear, dim3(grid_of(rows * ldw)), dim3(kBlock), 0, s, W, rows * ldw,
@@ -210,7 +211,8 @@ bws_status bws_csr_to_bitmap(const int64_t* csr_row_ptr, const int32_t* col_idx,
                              size_t workspace_bytes, bws_stream stream) {
   if (!shape_ok(rows, cols, ldw) || !n_invalid) return BWS_EINVAL;
   if (rows > 0 && (!W || !csr_row_ptr)) return BWS_EINVAL;
-  if (!workspace || workspace_bytes < bws_convert_workspace_size(rows, cols)) return BWS_EWORKSPACE;
+  if (!ws_ok(workspace, workspace_bytes, bws_convert_workspace_size(rows, cols)))
+    return BWS_EWORKSPACE;
   hipStream_t s = stream;
   const ConvWs c = carve_conv(workspace, rows, cols);
   hipLaunchKernelGGL(k_clear, dim3(grid_of(rows * ldw)), dim3(kBlock), 0, s, W, rows * ldw,
@@ -238,7 +240,7 @@ bws_status bws_bitmap_to_csr(const bws_bitmap* A, int64_t* row_ptr, int32_t* col
   BWS_TRY(check_bitmap(A, false));
   if (!row_ptr) return BWS_EINVAL;
   if (A->rows > INT32_MAX || A->cols > INT32_MAX) return BWS_ECAPACITY;
-  if (!workspace || workspace_bytes < bws_convert_workspace_size(A->rows, A->cols))
+  if (!ws_ok(workspace, workspace_bytes, bws_convert_workspace_size(A->rows, A->cols)))
     return BWS_EWORKSPACE;
   hipStream_t s = stream;
   const ConvWs c = carve_conv(workspace, A->rows, A->cols);

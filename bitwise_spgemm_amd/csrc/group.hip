@@ -244,8 +244,8 @@ bws_status bws_group_spilled_csr(const bws_bitmap* A, int32_t tile_rows,
   if (!spill_row_off || !spill_nnz_off) return BWS_EINVAL;
   if (tiles > 0 && (!spill_cnt || !spill_nnz || !spill_table || !row_end || !col || !val))
     return BWS_EINVAL;
-  if (!workspace ||
-      workspace_bytes < bws_group_spilled_workspace_size(A->rows, A->cols, tile_rows))
+  if (!ws_ok(workspace, workspace_bytes,
+               bws_group_spilled_workspace_size(A->rows, A->cols, tile_rows)))
     return BWS_EWORKSPACE;
   const int64_t wpr = words_for(A->cols);
   Carve c(workspace);

@@ -450,7 +450,7 @@ bws_status bws_row_ptr(const uint32_t* W, int64_t rows, int64_t cols, int64_t ld
                        bws_stream stream) {
   if (!dims_ok(rows, cols) || !ldw_ok(cols, ldw) || !row_ptr) return BWS_EINVAL;
   if (rows > 0 && !W) return BWS_EINVAL;
-  if (!workspace || workspace_bytes < bws_row_ptr_workspace_size(rows)) return BWS_EWORKSPACE;
+  if (!ws_ok(workspace, workspace_bytes, bws_row_ptr_workspace_size(rows))) return BWS_EWORKSPACE;
   return launch_row_ptr(W, rows, cols, ldw, row_ptr, workspace, stream);
 }
 
@@ -493,7 +493,7 @@ bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
   BWS_TRY(check_bitmap(A, true));
   const int64_t rows = A->rows, cols = A->cols;
   if (!ldw_ok(rows, ldwt) || !WT || !row_ptr_T) return BWS_EINVAL;
-  if (!workspace || workspace_bytes < bws_transpose_workspace_size(rows, cols))
+  if (!ws_ok(workspace, workspace_bytes, bws_transpose_workspace_size(rows, cols)))
     return BWS_EWORKSPACE;
   const int64_t wpr = words_for(cols), wprT = words_for(rows);
   Carve c(workspace);

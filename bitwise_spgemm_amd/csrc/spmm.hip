@@ -204,7 +204,7 @@ bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t
   if (precision == BWS_PREC_FP32)
     return sddmm_f32_launch(A, static_cast<const float*>(dC), lddc,
                             static_cast<const float*>(B), ldb, N, dA_values, stream);
-  if (!workspace || workspace_bytes < sddmm_mfma_workspace(A)) return BWS_EWORKSPACE;
+  if (!ws_ok(workspace, workspace_bytes, sddmm_mfma_workspace(A))) return BWS_EWORKSPACE;
   return sddmm_mfma_launch(A, dC, B, dtype, lddc, ldb, N, dA_values, precision, workspace, stream);
 }
 

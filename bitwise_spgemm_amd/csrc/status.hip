@@ -29,7 +29,7 @@ const char* bws_status_string(bws_status status) {
     case BWS_ECAPACITY: return "BWS_ECAPACITY: size exceeds a documented capacity";
     case BWS_EUNSUPPORTED: return "BWS_EUNSUPPORTED: unsupported dtype/precision combination";
     case BWS_EHIP: return "BWS_EHIP: a HIP call failed";
-    case BWS_EWORKSPACE: return "BWS_EWORKSPACE: workspace missing or too small";
+    case BWS_EWORKSPACE: return "BWS_EWORKSPACE: workspace missing, too small or not 256-byte aligned";
     case BWS_ENODEVICE: return "BWS_ENODEVICE: no usable gfx950 device";
     case BWS_ENOTFOUND: return "BWS_ENOTFOUND: no cached autotuner entry";
     default: return "BWS_UNKNOWN";

@@ -21,7 +21,8 @@
  *   - Dense matrices are row-major with a leading dimension in elements.
  *   - All device pointers are caller-owned; the library never allocates device
  *     memory on any entry point (workspace is passed in, sized by the matching
- *     *_workspace_size query).  All work is enqueued on the caller's stream with
+ *     *_workspace_size query, 256-byte aligned as hipMalloc returns it;
+ *     BWS_EWORKSPACE otherwise).  All work is enqueued on the caller's stream with
  *     no host synchronisation, so every entry point can be captured into a
  *     hipGraph.
  *   - Every entry point returns a bws_status (0 = BWS_OK, negative = error); no

@@ -52,6 +52,8 @@ def test_pack_validation(bws):
     assert L.bws_unpack_bitplanes_f32(FAKE, 10, 0, FAKE, None) == bws.BWS_EINVAL
     assert L.bws_row_ptr(FAKE, 10, 64, 2, FAKE, None, 0, None) == bws.BWS_EWORKSPACE
     assert L.bws_row_ptr(FAKE, 10, 64, 2, FAKE, FAKE, 1, None) == bws.BWS_EWORKSPACE
+    ws = L.bws_row_ptr_workspace_size(10)
+    assert L.bws_row_ptr(FAKE, 10, 64, 2, FAKE, FAKE + 8, ws, None) == bws.BWS_EWORKSPACE  # alignment
 
 
 def test_transpose_validation(bws):
@@ -152,6 +154,7 @@ def test_cluster_validation(bws):
     assert L.bws_cluster_rows(C.byref(a), -1, FAKE, FAKE, FAKE, ws, None) == bws.BWS_EINVAL
     assert L.bws_cluster_rows(C.byref(a), 0, None, FAKE, FAKE, ws, None) == bws.BWS_EINVAL
     assert L.bws_cluster_rows(C.byref(a), 0, FAKE, FAKE, FAKE, ws - 1, None) == bws.BWS_EWORKSPACE
+    assert L.bws_cluster_rows(C.byref(a), 0, FAKE, FAKE, FAKE + 16, ws, None) == bws.BWS_EWORKSPACE
     assert L.bws_cluster_rows(C.byref(_bm(0, 64, 2)), 0, None, None, None, 0, None) == bws.BWS_OK
     assert L.bws_cluster_rows(C.byref(_bm(1 << 30, 64, 2)), 0, FAKE, FAKE, FAKE, ws,
                               None) == bws.BWS_ECAPACITY
