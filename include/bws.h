@@ -23,8 +23,10 @@
  *     memory on any entry point (workspace is passed in, sized by the matching
  *     *_workspace_size query, 256-byte aligned as hipMalloc returns it;
  *     BWS_EWORKSPACE otherwise).  All work is enqueued on the caller's stream with
- *     no host synchronisation, so every entry point can be captured into a
- *     hipGraph.
+ *     no host synchronisation, so the entry points can be captured into a
+ *     hipGraph, except the set-up calls bws_batch_prepare and bws_autotune
+ *     (they copy or time on the host) and bws_cluster_rows (its row sort may
+ *     record memset nodes).
  *   - Every entry point returns a bws_status (0 = BWS_OK, negative = error); no
  *     entry point aborts or prints.  Argument validation happens on the host
  *     before any HIP call, so invalid calls are rejected without a GPU.
