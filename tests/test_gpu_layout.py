@@ -53,7 +53,8 @@ def test_bitplanes(oracle, bws, dev, rng):
 
 
 @pytest.mark.parametrize("rows,cols,p", [(1, 5, 1.0), (300, 300, 0.05), (1000, 2000, 0.01),
-                                         (4096, 100, 0.3), (0, 64, 0.1)])
+                                         (4096, 100, 0.3), (0, 64, 0.1), (257, 4100, 0.5),
+                                         (600, 256, 0.9)])
 def test_row_ptr_values(oracle, bws, dev, rng, rows, cols, p):
     D = rng.standard_normal((rows, cols)).astype(np.float32)
     D[rng.random(D.shape) >= p] = 0
@@ -68,7 +69,8 @@ def test_row_ptr_values(oracle, bws, dev, rng, rows, cols, p):
 
 
 @pytest.mark.parametrize("rows,cols,p", [(45, 70, 0.3), (64, 64, 0.1), (1, 1, 1.0),
-                                         (513, 300, 0.02), (100, 33, 0.5)])
+                                         (513, 300, 0.02), (100, 33, 0.5), (700, 1000, 0.4),
+                                         (130, 96, 1.0)])
 def test_transpose(oracle, bws, dev, rng, rows, cols, p):
     D = rng.standard_normal((rows, cols)).astype(np.float32)
     D[rng.random(D.shape) >= p] = 0
