@@ -118,6 +118,8 @@ _SIGS = {
     "bws_transpose_bitmap": ([_P, _I64, _I64, _I64, _P, _I64, _P], _I32),
     "bws_transpose_workspace_size": ([_I64, _I64], _SZ),
     "bws_transpose_values": ([_BP, _P, _I64, _P, _P, _P, _SZ, _P], _I32),
+    "bws_transpose_perm": ([_BP, _P, _I64, _P, _P, _P, _SZ, _P], _I32),
+    "bws_gather_f32": ([_P, _P, _I64, _P, _P], _I32),
     "bws_convert_workspace_size": ([_I64, _I64], _SZ),
     "bws_coo_to_bitmap": ([_P, _P, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P], _I32),
     "bws_csr_to_bitmap": ([_P, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P], _I32),
@@ -377,6 +379,36 @@ def transpose(A: Bitmap, with_values: bool = True) -> Bitmap:
     _check(lib().bws_transpose_values(C.byref(s), _ptr(WT), ldwt, _ptr(rpt), _ptr(vt), _ptr(ws),
                                       ws.numel(), _stream(dev)), "bws_transpose_values")
     return Bitmap(WT, A.cols, A.rows, rpt, None if vt is None else vt[:nnz])
+
+
+def transpose_perm(A: Bitmap):
+    """Aᵀ's words and row pointers plus perm (int64 [nnz]): AT's j-th value is
+    A's values[perm[j]].  For a fixed mask whose values change, AT's values are
+    then gather(values, perm)."""
+    dev = A.words.device
+    if A.row_ptr is None:
+        A = A.with_values(A.values, row_ptr(A))
+    ldwt = words_for(A.rows)
+    WT = torch.empty((A.cols, ldwt), dtype=torch.int32, device=dev)
+    nnz = int(A.row_ptr[-1].item())
+    rpt = torch.empty(A.cols + 1, dtype=torch.int64, device=dev)
+    perm = torch.empty(max(nnz, 1), dtype=torch.int64, device=dev)
+    ws = _workspace(lib().bws_transpose_workspace_size(A.rows, A.cols), dev)
+    s = A.struct()
+    _check(lib().bws_transpose_perm(C.byref(s), _ptr(WT), ldwt, _ptr(rpt), _ptr(perm), _ptr(ws),
+                                    ws.numel(), _stream(dev)), "bws_transpose_perm")
+    return Bitmap(WT, A.cols, A.rows, rpt), perm[:nnz]
+
+
+def gather(src: torch.Tensor, idx: torch.Tensor, out: Optional[torch.Tensor] = None):
+    """out[i] = src[idx[i]] (FP32 values, int64 indices) on the device."""
+    _require_gpu(src, idx)
+    assert src.dtype == torch.float32 and idx.dtype == torch.int64
+    if out is None:
+        out = torch.empty(idx.numel(), dtype=torch.float32, device=src.device)
+    _check(lib().bws_gather_f32(_ptr(src), _ptr(idx), idx.numel(), _ptr(out),
+                                _stream(src.device)), "bws_gather_f32")
+    return out
 
 
 # ----------------------------------------------------------------------------

@@ -282,13 +282,16 @@ k_transpose_bitmap(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, i
 // Values of Aᵀ: one thread per word of A.  A word's set bits are taken up to
 // four at a time so that their (independent) reads of WT, the word offsets
 // and row pointers of Aᵀ are in flight together.
+// PERM: write the source index (perm_T[dst] = src) instead of the value.
+template <bool PERM>
 __global__ void __launch_bounds__(kBlock)
 k_transpose_values(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t wpr,
                    int64_t ldw,
                    const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ woffA,
                    const float* __restrict__ values, const uint32_t* __restrict__ WT,
                    int64_t wprT, int64_t ldwt, const int64_t* __restrict__ row_ptr_T,
-                   const int32_t* __restrict__ woffT, float* __restrict__ values_T) {
+                   const int32_t* __restrict__ woffT, float* __restrict__ values_T,
+                   int64_t* __restrict__ perm_T) {
   const int64_t n = rows * wpr;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * kBlock) {
@@ -320,15 +323,28 @@ k_transpose_values(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, i
         if (j < cnt) {
           tw[j] = WT[c[j] * ldwt + x];
           base[j] = row_ptr_T[c[j]] + woffT[c[j] * wprT + x];
-          v[j] = values ? values[src + j] : 1.0f;
+          if (!PERM) v[j] = values ? values[src + j] : 1.0f;
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j < cnt) values_T[base[j] + rank_before(tw[j], rb)] = v[j];
+      for (int j = 0; j < 4; ++j) {
+        if (j >= cnt) continue;
+        const int64_t dst = base[j] + rank_before(tw[j], rb);
+        if (PERM) perm_T[dst] = src + j;
+        else values_T[dst] = v[j];
+      }
       src += cnt;
     }
   }
+}
+
+// dst[i] = src[idx[i]].
+__global__ void __launch_bounds__(kBlock)
+k_gather_f32(const float* __restrict__ src, const int64_t* __restrict__ idx, int64_t n,
+             float* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[idx[i]];
 }
 
 inline int grid_for(int64_t items, int64_t per_block) {
@@ -516,9 +532,11 @@ size_t bws_transpose_workspace_size(int64_t rows, int64_t cols) {
   return c.used + 256;
 }
 
-bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
-                                int64_t* row_ptr_T, float* values_T, void* workspace,
-                                size_t workspace_bytes, bws_stream stream) {
+// Shared by bws_transpose_values and bws_transpose_perm: AT's words and row
+// pointers, then (values_T or perm_T non-null) its values or their sources.
+static bws_status transpose_impl(const bws_bitmap* A, uint32_t* WT, int64_t ldwt, int64_t* row_ptr_T,
+                          float* values_T, int64_t* perm_T, void* workspace,
+                          size_t workspace_bytes, hipStream_t stream) {
   BWS_TRY(check_bitmap(A, true));
   const int64_t rows = A->rows, cols = A->cols;
   if (!ldw_ok(rows, ldwt) || !WT || !row_ptr_T) return BWS_EINVAL;
@@ -531,12 +549,43 @@ bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
   void* scan_ws = c.take<char>(scan_workspace_bytes(cols));
   BWS_TRY(bws_transpose_bitmap(A->words, rows, cols, A->ldw, WT, ldwt, stream));
   BWS_TRY(launch_row_ptr(WT, cols, rows, ldwt, row_ptr_T, scan_ws, stream));
-  if (!values_T || rows == 0 || cols == 0) return BWS_OK;
+  if ((!values_T && !perm_T) || rows == 0 || cols == 0) return BWS_OK;
   BWS_TRY(launch_word_offsets(A->words, rows, cols, A->ldw, woffA, stream));
   BWS_TRY(launch_word_offsets(WT, cols, rows, ldwt, woffT, stream));
-  hipLaunchKernelGGL(k_transpose_values, dim3(grid_for(rows * wpr, kBlock)), dim3(kBlock), 0,
-                     stream, A->words, rows, cols, wpr, A->ldw, A->row_ptr, woffA, A->values, WT, wprT,
-                     ldwt, row_ptr_T, woffT, values_T);
+  const dim3 grid(grid_for(rows * wpr, kBlock));
+  if (perm_T)
+    hipLaunchKernelGGL(k_transpose_values<true>, grid, dim3(kBlock), 0, stream, A->words, rows,
+                       cols, wpr, A->ldw, A->row_ptr, woffA, nullptr, WT, wprT, ldwt, row_ptr_T,
+                       woffT, nullptr, perm_T);
+  else
+    hipLaunchKernelGGL(k_transpose_values<false>, grid, dim3(kBlock), 0, stream, A->words, rows,
+                       cols, wpr, A->ldw, A->row_ptr, woffA, A->values, WT, wprT, ldwt,
+                       row_ptr_T, woffT, values_T, nullptr);
+  return launch_check();
+}
+
+bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
+                                int64_t* row_ptr_T, float* values_T, void* workspace,
+                                size_t workspace_bytes, bws_stream stream) {
+  return transpose_impl(A, WT, ldwt, row_ptr_T, values_T, nullptr, workspace, workspace_bytes,
+                        stream);
+}
+
+bws_status bws_transpose_perm(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
+                              int64_t* row_ptr_T, int64_t* perm_T, void* workspace,
+                              size_t workspace_bytes, bws_stream stream) {
+  if (!perm_T) return BWS_EINVAL;
+  return transpose_impl(A, WT, ldwt, row_ptr_T, nullptr, perm_T, workspace, workspace_bytes,
+                        stream);
+}
+
+bws_status bws_gather_f32(const float* src, const int64_t* idx, int64_t n, float* dst,
+                          bws_stream stream) {
+  if (n < 0) return BWS_EINVAL;
+  if (n == 0) return BWS_OK;
+  if (!src || !idx || !dst) return BWS_EINVAL;
+  hipLaunchKernelGGL(k_gather_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, src, idx,
+                     n, dst);
   return launch_check();
 }
 

@@ -166,6 +166,16 @@ size_t bws_transpose_workspace_size(int64_t rows, int64_t cols);
 bws_status bws_transpose_values(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
                                 int64_t* row_ptr_T, float* values_T, void* workspace,
                                 size_t workspace_bytes, bws_stream stream);
+/* The same transpose with the value gather left for later: perm_T[j] is the
+ * index in A's values of AT's j-th value (perm_T has nnz entries; A.values
+ * is ignored).  When the mask stays fixed and only its values change (a
+ * trained mask), AT's values are then bws_gather_f32(values, perm_T, nnz). */
+bws_status bws_transpose_perm(const bws_bitmap* A, uint32_t* WT, int64_t ldwt,
+                              int64_t* row_ptr_T, int64_t* perm_T, void* workspace,
+                              size_t workspace_bytes, bws_stream stream);
+/* dst[i] = src[idx[i]] for i < n. */
+bws_status bws_gather_f32(const float* src, const int64_t* idx, int64_t n, float* dst,
+                          bws_stream stream);
 
 /* Format converters (replace the serial tiling pass csr2tiledcsr,
  * source/src/main.cu:499-588, and cuSPARSE DenseToSparse, main.cu:630-690).

@@ -144,6 +144,20 @@ def test_sddmm_and_groups_validation(bws):
     assert L.bws_batch_prepare(arr, 1, 0, 0, None, FAKE, ws, C.byref(b)) == bws.BWS_EINVAL
 
 
+def test_transpose_perm_gather_validation(bws):
+    L = bws.lib()
+    a = _bm(40, 64, 2, row_ptr=FAKE)
+    ws = L.bws_transpose_workspace_size(40, 64)
+    assert L.bws_transpose_perm(C.byref(a), FAKE, 2, FAKE, None, FAKE, ws, None) == bws.BWS_EINVAL
+    assert L.bws_transpose_perm(C.byref(_bm(40, 64, 2)), FAKE, 2, FAKE, FAKE, FAKE, ws,
+                                None) == bws.BWS_EINVAL          # needs row_ptr
+    assert L.bws_transpose_perm(C.byref(a), FAKE, 2, FAKE, FAKE, FAKE, ws - 1,
+                                None) == bws.BWS_EWORKSPACE
+    assert L.bws_gather_f32(FAKE, FAKE, -1, FAKE, None) == bws.BWS_EINVAL
+    assert L.bws_gather_f32(None, FAKE, 5, FAKE, None) == bws.BWS_EINVAL
+    assert L.bws_gather_f32(None, None, 0, None, None) == bws.BWS_OK
+
+
 def test_cluster_validation(bws):
     L = bws.lib()
     a = _bm(64, 64, 2)

@@ -86,3 +86,26 @@ def test_transpose(oracle, bws, dev, rng, rows, cols, p):
     assert np.array_equal(to_host_u32(ATv.words), WT)
     assert np.array_equal(ATv.row_ptr.cpu().numpy(), rpt)
     assert np.array_equal(ATv.values.cpu().numpy(), vt)
+
+
+@pytest.mark.parametrize("rows,cols,p", [(45, 70, 0.3), (513, 300, 0.02), (700, 1000, 0.4),
+                                         (1, 1, 1.0)])
+def test_transpose_perm_and_gather(oracle, bws, dev, rng, rows, cols, p):
+    """perm_T maps AT's value order to A's: it equals the oracle's transpose of
+    the values 0, 1, 2, ...; gathering A's values through it gives AT's values."""
+    D = rng.standard_normal((rows, cols)).astype(np.float32)
+    D[rng.random(D.shape) >= p] = 0
+    W = oracle.pack_bitmap(D)
+    rp = oracle.row_ptr(W, cols)
+    nnz = int(rp[-1])
+    vals = oracle.pack_values(D, W, rp)
+    WT, rpt, vt = oracle.transpose_values(W, rows, cols, rp, vals)
+    _, _, order = oracle.transpose_values(W, rows, cols, rp, np.arange(nnz, dtype=np.float32))
+    A = bws.Bitmap(to_dev(W, dev), rows, cols, to_dev(rp, dev))
+    AT, perm = bws.transpose_perm(A)
+    assert np.array_equal(to_host_u32(AT.words), WT)
+    assert np.array_equal(AT.row_ptr.cpu().numpy(), rpt)
+    assert np.array_equal(perm.cpu().numpy(), order.astype(np.int64))
+    vT = bws.gather(to_dev(vals, dev), perm)
+    assert np.array_equal(vT.cpu().numpy(), vt)
+
