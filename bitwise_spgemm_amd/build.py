@@ -37,6 +37,7 @@ FILE_FLAGS = {
     "spmm_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
     # 128 x 128 wave tiles: 256 accumulators live in AGPRs
     "spmm_mfma_big.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=0"],
+    "sddmm_mfma_big.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=0"],
 }
 
 

@@ -92,6 +92,12 @@ bws_status launch_mfma4_big_batch(int32_t precision, const BatchEntry* entries,
                                   const int64_t* tile_off, int count, int64_t total_tiles,
                                   hipStream_t s);
 
+// the 256 x 256-block 16-bit MFMA SDDMM (sddmm_mfma_big.hip)
+bws_status launch_sddmm_big(int32_t precision, const bws_bitmap* A, const int32_t* woff,
+                            const void* dC, int64_t lddc, const void* B, int64_t ldb, int64_t N,
+                            float* dA, hipStream_t s);
+int64_t sddmm_big_tiles(int64_t rows, int64_t cols);
+
 // Device scratch carving: a linear allocator over the caller's workspace.
 struct Carve {
   char* base;

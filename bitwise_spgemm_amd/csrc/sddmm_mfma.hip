@@ -10,6 +10,8 @@
 // skipped as a whole.  The epilogue stores only set bits, at the value index
 // row_ptr[r] + word offset + popcount of the earlier bits of the word.
 // BF16X2 splits both operands (hi + lo) and accumulates hi·hi + hi·lo + lo·hi.
+// Large problems with direct-to-LDS operands take the 256 x 256-block kernel
+// of sddmm_mfma_big.hip.
 #include "mfma_common.hpp"
 
 namespace bws {
@@ -363,12 +365,16 @@ bws_status sddmm_mfma_launch(const bws_bitmap* A, const void* dC, const void* B,
                              void* ws, hipStream_t s) {
   int32_t* woff = static_cast<int32_t*>(ws);
   BWS_TRY(launch_word_offsets(A->words, A->rows, A->cols, A->ldw, woff, s));
-  const bool dma = dtype != BWS_DTYPE_F32 && precision != BWS_PREC_BF16X2 && N >= NSTG &&
-                   N % NSTG == 0 &&
+  // direct-to-LDS staging: 16-bit operands in the precision's own type, rows
+  // 16-byte aligned
+  const bool dma = ((precision == BWS_PREC_BF16 && dtype == BWS_DTYPE_BF16) ||
+                    (precision == BWS_PREC_FP16 && dtype == BWS_DTYPE_F16)) &&
                    lddc % 8 == 0 && ldb % 8 == 0 && reinterpret_cast<uintptr_t>(dC) % 16 == 0 &&
                    reinterpret_cast<uintptr_t>(B) % 16 == 0;
-  if (dma && ((precision == BWS_PREC_BF16 && dtype == BWS_DTYPE_BF16) ||
-              (precision == BWS_PREC_FP16 && dtype == BWS_DTYPE_F16))) {
+  // 256 x 256 blocks once there are enough of them to occupy most CUs
+  if (dma && N >= 32 && N % 32 == 0 && sddmm_big_tiles(A->rows, A->cols) >= 128)
+    return launch_sddmm_big(precision, A, woff, dC, lddc, B, ldb, N, dA, s);
+  if (dma && N >= NSTG && N % NSTG == 0) {
     const int64_t row_blocks = ceil_div(A->rows, BR), k_blocks = ceil_div(A->cols, BK);
     const int64_t per_xcd = ceil_div(row_blocks * k_blocks, 8);
     if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;

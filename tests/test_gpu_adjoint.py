@@ -22,7 +22,8 @@ def sddmm_err(oracle, W, K, rp, dC, B, got):
 @pytest.mark.parametrize("M,K,N,p", [(128, 128, 64, 0.1), (300, 500, 200, 0.05), (17, 33, 3, 0.5),
                                      (1024, 512, 1000, 0.02), (64, 64, 32, 1.0),
                                      (1000, 300, 256, 0.05), (129, 1000, 64, 0.3),
-                                     (512, 512, 2048, 0.1)])
+                                     (512, 512, 2048, 0.1), (4096, 2048, 96, 0.05),
+                                     (4000, 2100, 64, 0.1)])
 @pytest.mark.parametrize("prec,dt", [("bf16", torch.bfloat16), ("bf16", torch.float32),
                                      ("fp16", torch.float16), ("fp16", torch.float32),
                                      ("bf16x2", torch.float32)])
@@ -40,6 +41,23 @@ def test_sddmm_mfma(oracle, bws, dev, M, K, N, p, prec, dt):
     assert sddmm_err(oracle, W, K, rp, rnd(dC), rnd(B), got) <= 2e-5
     bound = 1e-2 if prec == "bf16" else 2e-3
     assert sddmm_err(oracle, W, K, rp, dC, B, got) <= bound
+
+
+@pytest.mark.parametrize("M,K,N", [(4096, 2048, 128), (4000, 2100, 96), (300, 500, 128)])
+def test_sddmm_mfma_kernels_agree(bws, dev, M, K, N):
+    """16-bit operands (direct-to-LDS kernels: 256 x 256 blocks for the large
+    shapes, 128 x 128 for the small one) and FP32 operands rounded on the
+    device (register-staged kernel) run the same MFMA sequence per output:
+    the results must be identical."""
+    W = bws.gen_bitmap_iid(M + K, M, K, 0.05)
+    A0 = bws.Bitmap(to_dev(W, dev), M, K)
+    A = A0.with_values(None, bws.row_ptr(A0))
+    dC = torch.from_numpy(bws.gen_f32(5, (M, N), "normal", 0.0, 1.0)).to(dev)
+    B = torch.from_numpy(bws.gen_f32(6, (K, N), "normal", 0.0, 1.0)).to(dev)
+    for prec, dt in (("bf16", torch.bfloat16), ("fp16", torch.float16)):
+        x = bws.sddmm(A, dC.to(dt), B.to(dt), prec)
+        y = bws.sddmm(A, dC, B, prec)
+        assert torch.equal(x, y), prec
 
 
 def test_sddmm_fp32_strided(oracle, bws, dev):
