@@ -23,6 +23,48 @@ constexpr int LDN = NSTG + 8;  // 144-byte LDS rows: 16-byte aligned, conflict-f
 __device__ __forceinline__ uint32_t to_bf16(float x) { return f32_to_bf16_rne(x); }
 __device__ __forceinline__ uint32_t to_f16(float x) { return f32_to_f16_rne(x); }
 
+// Epilogue of a 128 x 128 block: its bitmap words and the value index of each
+// word's first bit are staged in LDS (`scratch`: 6 KiB the block no longer
+// needs), then every lane stores the set bits among its accumulators (rows
+// 4q + e of each 16-row tile, column j of each 16-column tile).
+__device__ __forceinline__ void store_set_bits_128(const f32x4 (&acc)[4][4], uint8_t* scratch,
+                                                   const uint32_t* __restrict__ W, int64_t M,
+                                                   int64_t K, int64_t ldw,
+                                                   const int64_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ woff,
+                                                   float* __restrict__ dA, int64_t r0,
+                                                   int64_t k0, int wm, int wn, int j, int q) {
+  const int64_t wpr = (K + 31) / 32;
+  uint32_t* eW = reinterpret_cast<uint32_t*>(scratch);          // [128][4] words
+  int64_t* eBase = reinterpret_cast<int64_t*>(scratch + 2048);  // [128][4] value indices
+  __syncthreads();  // every wave is done with the operand buffers
+  for (int t = threadIdx.x; t < BR * (BK / 32); t += 256) {
+    const int64_t r = r0 + (t >> 2), w = k0 / 32 + (t & 3);
+    uint32_t word = 0u;
+    int64_t base = 0;
+    if (r < M && w < wpr) {
+      word = W[r * ldw + w] & valid_bits(w, K);
+      base = row_ptr[r] + woff[r * wpr + w];
+    }
+    eW[t] = word;
+    eBase[t] = base;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int rr = 64 * wm + 16 * a + 4 * q + e;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int ew = rr * (BK / 32) + 2 * wn + (b >> 1);
+        const uint32_t word = eW[ew];
+        const int i = 16 * (b & 1) + j;
+        if (word & wbit(i)) dA[eBase[ew] + rank_before(word, i)] = acc[a][b][e];
+      }
+    }
+}
+
 // Load 8 consecutive elements (row, n..n+7) as raw input, zero outside.
 template <int TB>
 struct Raw8;
@@ -198,24 +240,8 @@ k_sddmm_mfma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
     __syncthreads();
   }
 
-  // store the set bits only
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t r = r0 + 64 * wm + 16 * a + 4 * q + e;
-      if (r >= M) continue;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int64_t k = k0 + 64 * wn + 16 * b + j;
-        if (k >= K) continue;
-        const int64_t w = k >> 5;
-        const uint32_t word = W[r * ldw + w];
-        const int i = (int)(k & 31);
-        if (word & wbit(i))
-          dA[row_ptr[r] + woff[r * wpr + w] + rank_before(word, i)] = acc[a][b][e];
-      }
-    }
+  store_set_bits_128(acc, reinterpret_cast<uint8_t*>(&sC[0][0][0][0]), W, M, K, ldw, row_ptr,
+                     woff, dA, r0, k0, wm, wn, j, q);
 }
 
 // Direct-to-LDS variant for 16-bit dC and B (rows 16-byte aligned, leading
@@ -318,23 +344,7 @@ k_sddmm_mfma_dma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
     }
   }
 
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t r = r0 + 64 * wm + 16 * a + 4 * q + e;
-      if (r >= M) continue;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int64_t k = k0 + 64 * wn + 16 * b + j;
-        if (k >= K) continue;
-        const int64_t w = k >> 5;
-        const uint32_t word = W[r * ldw + w];
-        const int i = (int)(k & 31);
-        if (word & wbit(i))
-          dA[row_ptr[r] + woff[r * wpr + w] + rank_before(word, i)] = acc[a][b][e];
-      }
-    }
+  store_set_bits_128(acc, &sC[0][0], W, M, K, ldw, row_ptr, woff, dA, r0, k0, wm, wn, j, q);
 }
 
 template <int PREC, int TB>
