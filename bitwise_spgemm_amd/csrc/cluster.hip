@@ -75,6 +75,9 @@ k_cluster_iter(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t r = (int64_t)blockIdx.x * kDistWaves + wave;
   const int64_t wpr = words_for(K);
+  // the row's current nearest distance does not depend on the new leader:
+  // its load is issued first, beside the leader's
+  const int prev = (r < M && iter > 0) ? dist[r] : 0;
   const int64_t lead = iter == 0 ? 0 : (int64_t)(~(uint32_t)slots[iter - 1]);
   unsigned long long key = 0ull;
   if (r < M) {
@@ -82,16 +85,12 @@ k_cluster_iter(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw
     for (int64_t w = lane; w < wpr; w += 64)
       d += __popc((W[r * ldw + w] ^ W[lead * ldw + w]) & valid_bits(w, K));
     d = wave_sum(d);
-    int dd = d;
-    if (lane == 0) {
-      if (iter == 0 || d < dist[r]) {
-        dist[r] = d;
-        owner[r] = iter;
-      } else {
-        dd = dist[r];
-      }
+    const bool nearer = iter == 0 || d < prev;
+    if (lane == 0 && nearer) {
+      dist[r] = d;
+      owner[r] = iter;
     }
-    dd = __shfl(dd, 0, 64);
+    const int dd = nearer ? d : prev;
     key = ((unsigned long long)(uint32_t)dd << 32) | (uint32_t)(~(uint32_t)r);
   }
   if (lane == 0) best[wave] = key;
