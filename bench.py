@@ -309,10 +309,40 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     del AT, dC, dB, R, S
 
     if not args.quick:
+        ex["density_sweep"] = _density_sweep(bws, dev, M, N, K, args)
         ex["value_mask"] = _value_mask_workload(bws, dev)
         ex["grouped"] = _grouped_workload(bws, dev)
         ex["comparators"] = _comparators(bws, A, Bd, M, N, K, nnz, dev)
     return ex
+
+
+def _density_sweep(bws, dev, M, N, K, args):
+    """The clustered workload at nominal prototype densities 0.5 / 0.1 / 0.05 /
+    0.01 (measured density p·0.98 + (1-p)·0.02): FP32 plain and row-clustered
+    (with its plan's delta-row fraction) and BF16 MFMA forward times."""
+    import torch
+    out = []
+    B = torch.from_numpy(bws.gen_f32(args.seed + 1, (K, N), "normal", 0.0, 1.0)).to(dev)
+    Bb = B.to(torch.bfloat16)
+    C = torch.empty((M, N), device=dev)
+    def tl(f):
+        spin_up(f, 0.05)
+        return time_loop(f, 10, 2)[1]
+    for p in (0.5, 0.1, 0.05, 0.01):
+        W = bws.gen_bitmap_clustered(args.seed, M, K, 64, p, 0.02)
+        nnz = popcount_u32(W)
+        A = bws.from_host_words(W, M, K, dev)
+        pl = bws.cluster_rows(A)
+        delta = float(((pl.slot_row & bws.SLOT_DELTA) != 0).float().mean().item())
+        f32 = tl(lambda: bws.spmm(A, B, "fp32", out=C))
+        f32c = tl(lambda: bws.spmm_clustered(A, pl, B, out=C))
+        b16 = tl(lambda: bws.spmm(A, Bb, "bf16", out=C))
+        fl = 2.0 * nnz * N / 1e9
+        out.append({"p_proto": p, "density": round(nnz / (M * K), 4), "nnz": nnz,
+                    "fp32_ms": round(f32, 4), "fp32_tflops": round(fl / f32, 2),
+                    "fp32_rowclusters_ms": round(f32c, 4), "delta_rows": round(delta, 3),
+                    "bf16_ms": round(b16, 4), "bf16_tflops": round(fl / b16, 2)})
+    return out
 
 
 def _value_mask_workload(bws, dev):
