@@ -382,7 +382,7 @@ bws_status sddmm_mfma_launch(const bws_bitmap* A, const void* dC, const void* B,
                    lddc % 8 == 0 && ldb % 8 == 0 && reinterpret_cast<uintptr_t>(dC) % 16 == 0 &&
                    reinterpret_cast<uintptr_t>(B) % 16 == 0;
   // 256 x 256 blocks once there are enough of them to occupy most CUs
-  if (dma && N >= 32 && N % 32 == 0 && sddmm_big_tiles(A->rows, A->cols) >= 128)
+  if (dma && N >= 64 && N % 64 == 0 && sddmm_big_tiles(A->rows, A->cols) >= 128)
     return launch_sddmm_big(precision, A, woff, dC, lddc, B, ldb, N, dA, s);
   if (dma && N >= NSTG && N % NSTG == 0) {
     const int64_t row_blocks = ceil_div(A->rows, BR), k_blocks = ceil_div(A->cols, BK);

@@ -3,25 +3,34 @@
 //
 // Four waves of 128 x 128 (8 x 8 MFMA tiles, 256 FP32 accumulators each, held
 // in AGPRs: this file is compiled with the AGPR form of v_mfma), one wave per
-// SIMD.  Both operands are staged by global_load_lds_dwordx4, 32 values of N
-// per stage (64-byte rows, 16-byte chunk c of row r at c ^ ((r >> 2) & 3), so
-// the 16 lanes of a fragment read hit 16 different bank groups), four stages
-// in flight: the copy of stage s + 3 is issued while stage s is multiplied,
-// which covers the copies' latency at one wave per SIMD.  Per stage a wave
-// reads 8 + 8 fragments for 64 MFMAs (the 128 x 128-block kernel: 8 + 8 for
-// 32).  Each output's N steps run in ascending order, 32 at a time, as in the
-// other MFMA SDDMM kernels, so all of them give the same result.
+// SIMD.  Both operands are staged by global_load_lds_dwordx4, 64 values of N
+// per stage (128-byte rows, 16-byte chunk c of row r at c ^ ((r >> 1) & 7), so
+// the 16 lanes of a fragment read hit 16 different bank groups), two stages:
+// the copy of stage s + 1 is issued while stage s is multiplied (128 MFMAs
+// per wave).  Per 32-wide K step a wave reads 8 + 8 fragments for 64 MFMAs
+// (the 128 x 128-block kernel: 8 + 8 for 32).  Each output's N steps run in
+// ascending order, 32 at a time, as in the other MFMA SDDMM kernels, so all of
+// them give the same result.  (32-value stages with four in flight measured
+// 5% slower.)
 #include "mfma_common.hpp"
 
 namespace bws {
 namespace {
 
 constexpr int BT = 256;               // output block: BT rows of A x BT columns of A
-constexpr int NS = 32;                // N values per stage
-constexpr int NBUF = 4;               // stages in LDS
-constexpr int ROWB = NS * 2;          // 64 bytes per staged row
-constexpr int STAGE = BT * ROWB;      // 16 KiB per operand per stage
-constexpr int NC_W = STAGE / 1024 / 4;  // 1 KiB copies per wave per operand per stage (4)
+constexpr int NS = 64;                // N values per stage
+constexpr int NBUF = 2;               // stages in LDS
+constexpr int ROWB = NS * 2;          // bytes per staged row
+constexpr int CPR = ROWB / 16;        // 16-byte chunks per staged row
+constexpr int RPK = 1024 / ROWB;      // staged rows per 1 KiB copy
+constexpr int STAGE = BT * ROWB;      // bytes per operand per stage
+constexpr int NC_W = STAGE / 1024 / 4;  // 1 KiB copies per wave per operand per stage
+constexpr int KSTEPS = NS / 32;       // MFMA K steps per stage
+
+// LDS slot of 16-byte chunk c of staged row r: rows are swizzled so that the
+// 16 lanes reading one chunk of 16 consecutive rows hit 16 different bank
+// groups (64 banks x 4 bytes)
+__device__ __forceinline__ int slot_of(int r, int c) { return c ^ ((r / (16 / CPR)) % CPR); }
 
 template <int PREC>
 __global__ void __launch_bounds__(256, 1)
@@ -54,16 +63,17 @@ k_sddmm_mfma_big(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
     if (!__syncthreads_or(any != 0u)) return;
   }
 
-  // per-lane copy sources: instruction g of a stage fills LDS rows 16g ..
-  // 16g+15; lane l writes 16 bytes at slot l % 4 of row 16g + l / 4, which
-  // holds the row's global chunk (l % 4) ^ ((row >> 2) & 3)
+  // per-lane copy sources: instruction g of a stage fills LDS rows RPK g ..
+  // RPK g + RPK-1; lane l writes 16 bytes at slot l % CPR of row
+  // RPK g + l / CPR, which holds the row's global chunk slot_of(row, l % CPR)
+  // (the swizzle is its own inverse)
   const uint16_t* csrc[NC_W];
   const uint16_t* bsrc[NC_W];
 #pragma unroll
   for (int t = 0; t < NC_W; ++t) {
     const int g = wave * NC_W + t;
-    const int r = 16 * g + lane / 4;
-    const int c = (lane & 3) ^ ((r >> 2) & 3);
+    const int r = RPK * g + lane / CPR;
+    const int c = slot_of(r, lane % CPR);
     int64_t rc = r0 + r, rk = k0 + r;
     if (rc > M - 1) rc = M - 1;
     if (rk > K - 1) rk = K - 1;
@@ -91,29 +101,40 @@ k_sddmm_mfma_big(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
 #pragma unroll
   for (int p = 0; p < NBUF - 1; ++p)
     if (p < nst) issue(p);
-  // fragment offsets within a stage: row 16x + j, chunk q swizzled by row
-  const uint32_t rd_c = (uint32_t)((128 * wm + j) * ROWB + 16 * (q ^ ((j >> 2) & 3)));
-  const uint32_t rd_b = (uint32_t)((128 * wn + j) * ROWB + 16 * (q ^ ((j >> 2) & 3)));
+  // fragment rows 16x + j of the wave's 128 rows (the swizzle depends only on
+  // j since 16x is a multiple of 16)
+  const uint32_t row_c = (uint32_t)((128 * wm + j) * ROWB);
+  const uint32_t row_b = (uint32_t)((128 * wn + j) * ROWB);
 
   for (int64_t st = 0; st < nst; ++st) {
-    // stage st has landed when at most the two later stages' copies are out
-    if (st + 2 < nst) wait_vmcnt<2 * 2 * NC_W>();
-    else if (st + 1 < nst) wait_vmcnt<2 * NC_W>();
-    else wait_vmcnt<0>();
+    // stage st has landed when at most the later stages' copies are out
+    if constexpr (NBUF == 4) {
+      if (st + 2 < nst) wait_vmcnt<2 * 2 * NC_W>();
+      else if (st + 1 < nst) wait_vmcnt<2 * NC_W>();
+      else wait_vmcnt<0>();
+    } else {
+      static_assert(NBUF == 2, "two or four stages");
+      wait_vmcnt<0>();
+    }
     __syncthreads();  // every wave's copies of stage st; stage st-1's buffer is free
     if (st + NBUF - 1 < nst) issue(st + NBUF - 1);
-    const uint32_t buf = (uint32_t)(st & (NBUF - 1)) * STAGE;
-    uint4 af[8], bf[8];
+    const uint8_t* bufC = &sC[0][0] + (uint32_t)(st & (NBUF - 1)) * STAGE;
+    const uint8_t* bufB = &sB[0][0] + (uint32_t)(st & (NBUF - 1)) * STAGE;
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
-      af[a] = *reinterpret_cast<const uint4*>(&sC[0][0] + buf + rd_c + a * 16 * ROWB);
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+      const uint32_t chunk = (uint32_t)(16 * slot_of(j, 4 * ks + q));
+      uint4 af[8], bf[8];
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
-      bf[b] = *reinterpret_cast<const uint4*>(&sB[0][0] + buf + rd_b + b * 16 * ROWB);
+      for (int a = 0; a < 8; ++a)
+        af[a] = *reinterpret_cast<const uint4*>(bufC + row_c + chunk + a * 16 * ROWB);
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b)
+        bf[b] = *reinterpret_cast<const uint4*>(bufB + row_b + chunk + b * 16 * ROWB);
 #pragma unroll
-      for (int b = 0; b < 8; ++b) acc[a][b] = mfma<PREC>(af[a], bf[b], acc[a][b]);
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = mfma<PREC>(af[a], bf[b], acc[a][b]);
+    }
   }
 
   // Epilogue.  The block's bitmap words and the value index of each word's
@@ -153,7 +174,7 @@ k_sddmm_mfma_big(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
 }  // namespace
 
 // 16-bit dC and B with 16-byte aligned rows, leading dimensions multiples of 8
-// and N a multiple of 32 (the caller checks); PREC 1 = BF16, 2 = FP16.
+// and N a multiple of 64 (the caller checks); PREC 1 = BF16, 2 = FP16.
 bws_status launch_sddmm_big(int32_t precision, const bws_bitmap* A, const int32_t* woff,
                             const void* dC, int64_t lddc, const void* B, int64_t ldb, int64_t N,
                             float* dA, hipStream_t s) {
