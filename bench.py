@@ -8,7 +8,8 @@ launch over the resident bitmap and B (the reference's timed region is
 likewise the GEMM on pre-packed, pre-grouped operands).  For FP32 the step is
 the row-clustered product (bws_spmm_clustered) when the row-clustering plan,
 built once per bitmap and timed separately, sums at least half of the rows
-as differences from a chunk base; otherwise the plain bws_spmm.  Both
+as differences from a chunk base and the clustered kernel is the faster of
+the two on these operands; otherwise the plain bws_spmm.  Both
 kernels' times are reported.  Prints ONE JSON line with effective TFLOP/s =
 2·nnz·N / t, the HBM roofline of the kernel, and the reference CPU naive
 loop timed on a bounded row sample of the same workload.
@@ -162,8 +163,16 @@ def main():
         plan_ms = time_loop(lambda: bws.cluster_rows(A), 3, 1)[1]
         delta = float(((clusters.slot_row & bws.SLOT_DELTA) != 0).float().mean().item())
         plan_info = {"plan_ms": round(plan_ms, 4), "delta_row_fraction": round(delta, 4)}
-        if args.fp32_path == "clustered" or delta >= 0.5:
-            step = lambda: bws.spmm_clustered(A, clusters, Bin, out=C)
+        clustered = lambda: bws.spmm_clustered(A, clusters, Bin, out=C)
+        use = args.fp32_path == "clustered"
+        if args.fp32_path == "auto" and delta >= 0.5:
+            # tuned like any plan choice: the faster of the two kernels here
+            spin_up(plain, 0.1)
+            t_plain = time_loop(plain, 10, 2)[1]
+            spin_up(clustered, 0.1)
+            use = time_loop(clustered, 10, 2)[1] < t_plain
+        if use:
+            step = clustered
             path = "fp32_rowclusters"
     spin_up(step)
     ms_step, ms_kernel = time_loop(step, args.steps, args.warmup)
