@@ -27,12 +27,16 @@ def test_pack_bitmap(oracle, bws, dev, rng, rows, cols, ldw, p):
         assert np.array_equal(bws.unpack_bitmap(A).cpu().numpy(), oracle.unpack_bitmap(Wref, cols))
 
 
-def test_pack_strided_input(oracle, bws, dev, rng):
-    big = rng.standard_normal((50, 130)).astype(np.float32)
+@pytest.mark.parametrize("ld,c0,cols", [(130, 3, 100),    # unaligned rows: scalar kernel
+                                         (1036, 4, 1028),  # 16-byte rows: vector kernel, ragged span
+                                         (1100, 1, 1024),  # cols % 4 == 0 but rows misaligned
+                                         (2052, 0, 2048)])
+def test_pack_strided_input(oracle, bws, dev, rng, ld, c0, cols):
+    big = rng.standard_normal((50, ld)).astype(np.float32)
     big[rng.random(big.shape) < 0.7] = 0
-    Dd = to_dev(big, dev)[:, 3:103]          # ld = 130, cols = 100
+    Dd = to_dev(big, dev)[:, c0:c0 + cols]
     A = bws.pack_bitmap(Dd)
-    assert np.array_equal(to_host_u32(A.words), oracle.pack_bitmap(big[:, 3:103]))
+    assert np.array_equal(to_host_u32(A.words), oracle.pack_bitmap(big[:, c0:c0 + cols]))
 
 
 def test_bitplanes(oracle, bws, dev, rng):
