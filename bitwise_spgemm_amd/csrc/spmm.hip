@@ -49,6 +49,10 @@ bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,
                      const bws_plan* plan, hipStream_t s);
 size_t spmm_vals_workspace(const bws_bitmap* A);
+size_t spmm_split_workspace(int64_t K, int64_t N);
+bws_status spmm_mfma_split(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
+                           int64_t ldc, float alpha, float beta, const bws_plan* plan,
+                           void* workspace, hipStream_t s);
 bws_status spmm_mfma_vals(const bws_bitmap* A, const void* B, int32_t precision, int64_t ldb,
                           int64_t N, float* C, int64_t ldc, float alpha, float beta,
                           void* workspace, hipStream_t s);
@@ -129,9 +133,11 @@ extern "C" {
 size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision,
                                const bws_plan* plan) {
   (void)N;
-  if (!A || A->rows < 0 || A->cols < 0 || !A->values) return 0;
-  if (precision != BWS_PREC_BF16 && precision != BWS_PREC_FP16) return 0;
+  if (!A || A->rows < 0 || A->cols < 0 || N < 0) return 0;
   if (plan && plan->kernel == 1) return 0;
+  if (!A->values)
+    return precision == BWS_PREC_BF16X2 && N % 8 == 0 ? spmm_split_workspace(A->cols, N) : 0;
+  if (precision != BWS_PREC_BF16 && precision != BWS_PREC_FP16) return 0;
   return spmm_vals_workspace(A);
 }
 
@@ -162,6 +168,12 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
       workspace_bytes >= spmm_vals_workspace(A) &&
       (reinterpret_cast<uintptr_t>(workspace) & 255u) == 0)
     return spmm_mfma_vals(A, B, precision, ldb, N, C, ldc, alpha, beta, workspace, stream);
+  // Split-BF16 on a 0/1 A: with room for B's two BF16 planes, split B once and
+  // run the direct-to-LDS BF16 kernel twice; otherwise split in registers.
+  if (!A->values && A->cols > 0 && precision == BWS_PREC_BF16X2 && (!plan || plan->kernel != 1) &&
+      N % 8 == 0 && ws_ok(workspace, workspace_bytes, spmm_split_workspace(A->cols, N)))
+    return spmm_mfma_split(A, static_cast<const float*>(B), ldb, N, C, ldc, alpha, beta, plan,
+                           workspace, stream);
   return spmm_dispatch(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, plan, stream);
 }
 

@@ -758,4 +758,65 @@ bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_
   return BWS_EUNSUPPORTED;
 }
 
+namespace {
+
+// hi = bf16(b), lo = bf16(b - hi) (the split of pack_block), four columns per
+// thread (N % 8 == 0); VEC: B rows 16-byte aligned, one float4 load.
+template <bool VEC>
+__global__ void __launch_bounds__(256)
+k_split_bf16x2(const float* __restrict__ B, int64_t K, int64_t N, int64_t ldb,
+               uint2* __restrict__ hi, uint2* __restrict__ lo) {
+  const int64_t quads = N / 4, total = K * quads;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t k = i / quads, c = 4 * (i - k * quads);
+    const float* src = B + k * ldb + c;
+    float e[4];
+    if (VEC) {
+      const float4 v = *reinterpret_cast<const float4*>(src);
+      e[0] = v.x, e[1] = v.y, e[2] = v.z, e[3] = v.w;
+    } else {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) e[x] = src[x];
+    }
+    uint32_t h[4], l[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      h[x] = f32_to_bf16_rne(e[x]);
+      l[x] = f32_to_bf16_rne(e[x] - bf16_to_f32(h[x]));
+    }
+    hi[i] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+    lo[i] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+  }
+}
+
+size_t split_plane_bytes(int64_t K, int64_t N) { return (size_t)ceil_div(K * N * 2, 256) * 256; }
+
+}  // namespace
+
+size_t spmm_split_workspace(int64_t K, int64_t N) { return 2 * split_plane_bytes(K, N); }
+
+// Split-BF16 through the 16-bit kernels: B's hi and lo planes are written to
+// the workspace (row-major, ld N) and multiplied by the BF16 MFMA kernel in
+// two passes, C = alpha·A·hi + beta·C, then C = alpha·A·lo + C.  Each pass
+// stages B straight into LDS, which the register-staged split cannot.
+bws_status spmm_mfma_split(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
+                           int64_t ldc, float alpha, float beta, const bws_plan* plan,
+                           void* workspace, hipStream_t s) {
+  const int64_t K = A->cols;
+  uint16_t* hi = static_cast<uint16_t*>(workspace);
+  uint16_t* lo = reinterpret_cast<uint16_t*>(static_cast<char*>(workspace) + split_plane_bytes(K, N));
+  const int64_t quads = K * (N / 4);
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(quads, 256), 16384);
+  if (ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0)
+    hipLaunchKernelGGL(k_split_bf16x2<true>, dim3(grid), dim3(256), 0, s, B, K, N, ldb,
+                       reinterpret_cast<uint2*>(hi), reinterpret_cast<uint2*>(lo));
+  else
+    hipLaunchKernelGGL(k_split_bf16x2<false>, dim3(grid), dim3(256), 0, s, B, K, N, ldb,
+                       reinterpret_cast<uint2*>(hi), reinterpret_cast<uint2*>(lo));
+  BWS_TRY(launch_check());
+  BWS_TRY(spmm_mfma(A, hi, BWS_DTYPE_BF16, N, N, C, ldc, alpha, beta, BWS_PREC_BF16, plan, s));
+  return spmm_mfma(A, lo, BWS_DTYPE_BF16, N, N, C, ldc, alpha, 1.0f, BWS_PREC_BF16, plan, s);
+}
+
 }  // namespace bws

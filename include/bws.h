@@ -344,6 +344,14 @@ typedef struct {
  * precision/b_dtype combinations: FP32 with F32 (or exact BF16/F16 input);
  * BF16 with BF16 or F32 (rounded to nearest even); FP16 with F16 or F32;
  * BF16X2 with F32.
+ * Workspace (optional; size from bws_spmm_workspace_size, 0 when unused):
+ *   - value-carrying A at BF16/FP16 with 16-bit B: a dense 16-bit copy of A
+ *     for the MFMA kernel (without it the FP32 vector path runs);
+ *   - BF16X2 with a 0/1 A and N % 8 == 0: B's hi and lo BF16 planes, split
+ *     once, then multiplied by the direct-to-LDS BF16 kernel in two passes
+ *     (C = alpha·A·hi + beta·C, then C += alpha·A·lo); without it B is split
+ *     in registers inside one kernel.  The two forms differ by FP32 rounding
+ *     only (one more rounding of the partial sum).
  * Semantics of the FP32 add path of backup/layout_trans.cu:145-171 and the
  * numeric kernel spgemm_compute_1dthread_tcore_v3 (source/src/transform.cuh:
  * 2062-2248) with the operand roles of a bitmap A and a dense B. */

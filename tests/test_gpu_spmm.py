@@ -122,6 +122,38 @@ def test_spmm_mfma(oracle, bws, dev, M, K, N, p, prec):
         assert np.array_equal(C2, C)
 
 
+@pytest.mark.parametrize("M,K,N,p,beta", [(1000, 500, 264, 0.05, 0.0), (257, 97, 136, 0.3, 0.5),
+                                          (512, 1024, 1024, 0.1, 0.0), (33, 1000, 8, 0.2, -1.0)])
+def test_spmm_bf16x2_two_forms(oracle, bws, dev, M, K, N, p, beta):
+    """BF16X2 with the workspace (B split once into BF16 planes, two passes of
+    the BF16 kernel) and without it (split in registers, one kernel): both
+    within the split bound of the FP64 oracle, and within FP32 rounding of
+    each other (the two-pass form rounds the partial sum once more)."""
+    import ctypes as C
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M + N)
+    C0 = torch.randn((M, N), device=dev)
+    s = A.struct()
+    lib = bws.lib()
+    wsz = lib.bws_spmm_workspace_size(C.byref(s), N, bws.PRECISION["bf16x2"], None)
+    assert wsz >= 2 * K * N * 2
+    outs = []
+    for ws in (None, bws._workspace(wsz, dev)):
+        out = C0.clone()
+        st = lib.bws_spmm(C.byref(s), bws._ptr(Bd), 0, Bd.stride(0), N, bws._ptr(out),
+                          out.stride(0), C.c_float(0.75), C.c_float(beta), bws.PRECISION["bf16x2"],
+                          None, bws._ptr(ws), 0 if ws is None else ws.numel(), bws._stream(dev))
+        assert st == 0
+        outs.append(out)
+    torch.cuda.synchronize()
+    ref = 0.75 * oracle.spmm_f64(W, K, B) + beta * C0.double().cpu().numpy()
+    S = 0.75 * oracle.spmm_f64(W, K, np.abs(B)) + abs(beta) * np.abs(C0.double().cpu().numpy())
+    for out in outs:
+        e = np.abs(out.double().cpu().numpy() - ref) / np.maximum(S, 1e-30)
+        assert e.max() <= 1e-5, e.max()
+    d = (outs[0] - outs[1]).abs().double().cpu().numpy() / np.maximum(S, 1e-30)
+    assert d.max() <= 2e-6, d.max()
+
+
 TILES = {"bf16": [(128, 128), (128, 256), (256, 128), (256, 256)],
          "fp16": [(128, 128), (128, 256), (256, 128), (256, 256)],
          "bf16x2": [(128, 128), (256, 128)]}
