@@ -17,14 +17,17 @@ bool valu_cfg_valid(const ValuCfg& c) {
   return false;
 }
 
-// Default vector-add tile for an M x N output.
+// Default vector-add tile for an M x N output: the tallest row block that
+// still gives ~192+ workgroups (one per CU: the B stages take 128 KiB of LDS),
+// always 16 waves.  Measured on 4096 x 4096 x {1024, 2048, 4096}, 2048² x 4096,
+// 8192 x 4096 x 1024 and 16384 x 4096 x 512 with 5% clustered and 10% iid
+// masks: 16 waves beat 8 waves on the same rows by 25-45% (four waves per
+// SIMD hide the LDS and bit-scan latency of the per-row chains).
 ValuCfg default_valu_cfg(int64_t M, int64_t N) {
   ValuCfg c;
+  c.waves = 16;
   const int64_t tiles256 = ceil_div(M, 256) * ceil_div(N, 256);
-  if (tiles256 >= 192) {  // 256 x 256 tiles fill the 256 CUs
-    c.waves = 16;          // 16 waves x 16 rows: 4 waves per SIMD hide LDS latency
-    return c;
-  }
+  if (tiles256 >= 192) return c;  // 16 waves x 16 rows
   const int64_t tiles128 = ceil_div(M, 128) * ceil_div(N, 256);
   if (tiles128 >= 192) {
     c.rows = 128;
@@ -40,7 +43,7 @@ ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N) {
   ValuCfg c = default_valu_cfg(M, N);
   if (plan && plan->rows_per_wg) {
     c.rows = plan->rows_per_wg;
-    c.waves = plan->waves_per_wg ? plan->waves_per_wg : (c.rows == 256 ? 16 : 8);
+    c.waves = plan->waves_per_wg ? plan->waves_per_wg : 16;
     c.kstage = plan->k_stage ? plan->k_stage : (c.rows == 64 ? 32 : 64);
   }
   return c;

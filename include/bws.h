@@ -310,7 +310,7 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
  *                 2 = MFMA tiles (16-bit precisions only)
  *   rows_per_wg   vector-add path: rows of C per workgroup (64, 128 or 256);
  *                 MFMA path: rows of C per workgroup (0 = auto, 128 or 256)
- *   waves_per_wg  vector-add path: 8 or 16 (MFMA: always 4, pass 0)
+ *   waves_per_wg  vector-add path: 8 or 16 (0 = 16) (MFMA: always 4, pass 0)
  *   k_stage       vector-add path: K rows staged per LDS buffer (32 or 64)
  *                 (MFMA: fixed by the tile, pass 0)
  *   mfma_min_bits reserved for the density classifier (0 = default)
