@@ -366,8 +366,15 @@ bws_status launch_sddmm(const bws_bitmap* A, const int32_t* woff, const void* dC
 
 }  // namespace
 
+int64_t sddmm_f32_tiles(int64_t rows, int64_t cols);
+
+// Workspace of bws_sddmm: per-word popcount offsets (int32, rows x words),
+// then one byte per 64 x 64 tile for the FP32 tiled kernel.
+size_t sddmm_woff_bytes(const bws_bitmap* A) {
+  return align_up((size_t)(A->rows * words_for(A->cols)) * sizeof(int32_t), 256);
+}
 size_t sddmm_mfma_workspace(const bws_bitmap* A) {
-  return align_up((size_t)(A->rows * words_for(A->cols)) * sizeof(int32_t), 256) + 256;
+  return sddmm_woff_bytes(A) + align_up((size_t)sddmm_f32_tiles(A->rows, A->cols), 256) + 256;
 }
 
 bws_status sddmm_mfma_launch(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,

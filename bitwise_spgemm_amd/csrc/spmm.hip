@@ -62,12 +62,18 @@ bws_status spmm_mfma_vals(const bws_bitmap* A, const void* B, int32_t precision,
 bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
                               int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, hipStream_t s);
+bool sddmm_f32_tile_ok(const float* dC, int64_t lddc, const float* B, int64_t ldb, int64_t N);
+bws_status launch_sddmm_f32_tile(const bws_bitmap* A, const int32_t* woff, const float* dC,
+                                 int64_t lddc, const float* B, int64_t ldb, int64_t N, float* dA,
+                                 uint8_t* dense_tile, hipStream_t s);
 bws_status sddmm_f32_launch(const bws_bitmap* A, const float* dC, int64_t lddc, const float* B,
-                            int64_t ldb, int64_t N, float* dA, hipStream_t s);
+                            int64_t ldb, int64_t N, float* dA, const uint8_t* dense_tile,
+                            hipStream_t s);
 bws_status sddmm_mfma_launch(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
                              int64_t lddc, int64_t ldb, int64_t N, float* dA, int32_t precision,
                              void* ws, hipStream_t s);
 size_t sddmm_mfma_workspace(const bws_bitmap* A);
+size_t sddmm_woff_bytes(const bws_bitmap* A);
 
 namespace {
 
@@ -216,9 +222,23 @@ bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t
   if (A->rows == 0 || A->cols == 0) return BWS_OK;
   if (!dA_values) return BWS_EINVAL;
   if (N > 0 && (!dC || !B || lddc < N || ldb < N)) return BWS_EINVAL;
-  if (precision == BWS_PREC_FP32)
-    return sddmm_f32_launch(A, static_cast<const float*>(dC), lddc,
-                            static_cast<const float*>(B), ldb, N, dA_values, stream);
+  if (precision == BWS_PREC_FP32) {
+    const float* dc32 = static_cast<const float*>(dC);
+    const float* b32 = static_cast<const float*>(B);
+    // 64 x 64 tiles with both operands staged in LDS for the tiles with at
+    // least 64 set bits, when the workspace holds A's per-word offsets and the
+    // tile flags and the rows allow 16-byte copies; then the wave-per-row
+    // kernel takes the bits of the other tiles
+    if (sddmm_f32_tile_ok(dc32, lddc, b32, ldb, N) &&
+        ws_ok(workspace, workspace_bytes, sddmm_mfma_workspace(A))) {
+      int32_t* woff = static_cast<int32_t*>(workspace);
+      uint8_t* dense = static_cast<uint8_t*>(workspace) + sddmm_woff_bytes(A);
+      BWS_TRY(launch_word_offsets(A->words, A->rows, A->cols, A->ldw, woff, stream));
+      BWS_TRY(launch_sddmm_f32_tile(A, woff, dc32, lddc, b32, ldb, N, dA_values, dense, stream));
+      return sddmm_f32_launch(A, dc32, lddc, b32, ldb, N, dA_values, dense, stream);
+    }
+    return sddmm_f32_launch(A, dc32, lddc, b32, ldb, N, dA_values, nullptr, stream);
+  }
   if (!ws_ok(workspace, workspace_bytes, sddmm_mfma_workspace(A))) return BWS_EWORKSPACE;
   return sddmm_mfma_launch(A, dC, B, dtype, lddc, ldb, N, dA_values, precision, workspace, stream);
 }

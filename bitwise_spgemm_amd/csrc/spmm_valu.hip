@@ -523,16 +523,29 @@ k_spmm_groups(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
 }
 
 // FP32 SDDMM: dA[row_ptr[r] + j] = Σ_n dC[r, n] B[k_j, n].  One wave per row;
-// the row of dC is held in registers in 1024-column slices.
+// the row of dC is held in registers in 1024-column slices.  With dense_tile
+// (one byte per 64 x 64 tile, tile index (k / 64) * tile_rows + r / 64), the
+// bits of tiles flagged 1 are skipped (the tiled kernel computed them) and
+// rows with nothing left are not visited.
 template <bool VEC>
 __global__ void __launch_bounds__(256)
 k_sddmm_f32(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
             const int64_t* __restrict__ row_ptr, const float* __restrict__ dC, int64_t lddc,
-            const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ dA) {
+            const float* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ dA,
+            const uint8_t* __restrict__ dense_tile, int64_t tile_rows) {
   const int lane = lane_id();
   const int64_t wpr = words_for(K);
+  auto skip_word = [&](int64_t r, int64_t w) {
+    return dense_tile && dense_tile[(w >> 1) * tile_rows + (r >> 6)] != 0;
+  };
   for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < M;
        r += (int64_t)gridDim.x * 4) {
+    if (dense_tile) {
+      bool any = false;
+      for (int64_t w = lane; w < wpr; w += 64)
+        any |= !skip_word(r, w) && (W[r * ldw + w] & valid_bits(w, K)) != 0u;
+      if (!__any(any)) continue;
+    }
     const int64_t e0 = row_ptr[r];
     for (int64_t n0 = 0; n0 < N || n0 == 0; n0 += 1024) {
       float4 x[4];
@@ -540,8 +553,14 @@ k_sddmm_f32(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
       for (int q = 0; q < 4; ++q)
         x[q] = load_b4<0, VEC>(dC + r * lddc, 0, n0 + 256 * q + 4 * lane, 0, 1, N);
       int64_t e = e0;
+      uint64_t skipm = 0;  // words w0 .. w0+63 to skip, fetched a group at a time
       for (int64_t w = 0; w < wpr; ++w) {
+        if (dense_tile && (w & 63) == 0) skipm = __ballot(w + lane < wpr && skip_word(r, w + lane));
         uint32_t m = W[r * ldw + w] & valid_bits(w, K);
+        if ((skipm >> (w & 63)) & 1ull) {
+          e += __popc(m);
+          continue;
+        }
         while (m) {
           const int i = __clz((int)m);
           m &= ~wbit(i);
@@ -703,16 +722,20 @@ bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, cons
 }
 
 bws_status sddmm_f32_launch(const bws_bitmap* A, const float* dC, int64_t lddc, const float* B,
-                            int64_t ldb, int64_t N, float* dA, hipStream_t s) {
+                            int64_t ldb, int64_t N, float* dA, const uint8_t* dense_tile,
+                            hipStream_t s) {
   const bool vec = (ldb % 4 == 0) && (lddc % 4 == 0) &&
                    (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
                    (reinterpret_cast<uintptr_t>(dC) % 16 == 0);
+  const int64_t tile_rows = ceil_div(A->rows, 64);
   if (vec)
     hipLaunchKernelGGL(k_sddmm_f32<true>, dim3(grid_for(A->rows, 4)), dim3(256), 0, s, A->words,
-                       A->rows, A->cols, A->ldw, A->row_ptr, dC, lddc, B, ldb, N, dA);
+                       A->rows, A->cols, A->ldw, A->row_ptr, dC, lddc, B, ldb, N, dA, dense_tile,
+                       tile_rows);
   else
     hipLaunchKernelGGL(k_sddmm_f32<false>, dim3(grid_for(A->rows, 4)), dim3(256), 0, s, A->words,
-                       A->rows, A->cols, A->ldw, A->row_ptr, dC, lddc, B, ldb, N, dA);
+                       A->rows, A->cols, A->ldw, A->row_ptr, dC, lddc, B, ldb, N, dA, dense_tile,
+                       tile_rows);
   return launch_check();
 }
 

@@ -396,7 +396,11 @@ bws_status bws_spmm_tA(const bws_bitmap* AT, const void* dC, int32_t dc_dtype, i
  * is ignored.  dC and B share dtype `dtype`: F32 for FP32 (vector FMAs) and
  * BF16X2; BF16/F16 or F32 (rounded on the device) for BF16/FP16, which run on
  * MFMA over 16 x 16 output blocks with FP32 accumulate and store only the set
- * bits.  The 16-bit paths need the workspace (per-word popcount offsets). */
+ * bits.  The 16-bit paths need the workspace (per-word popcount offsets).
+ * FP32 uses it when given (and N % 64 == 0, lddc and ldb multiples of 4, dC
+ * and B 16-byte aligned): 64 x 64 tiles of A with at least 64 set bits are
+ * computed with both operands staged in LDS, the other bits by a wave per
+ * row; without it every row takes the wave-per-row kernel. */
 size_t bws_sddmm_workspace_size(const bws_bitmap* A);
 bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t dtype,
                      int64_t lddc, int64_t ldb, int64_t N, float* dA_values,

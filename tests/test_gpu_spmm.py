@@ -245,7 +245,12 @@ def test_spmm_tA(oracle, bws, dev, values):
     assert np.max(np.abs(dB - R) / np.maximum(S, 1e-30)) <= 1e-5
 
 
-@pytest.mark.parametrize("M,K,N,p", [(100, 80, 64, 0.1), (300, 500, 1500, 0.05), (17, 33, 3, 0.5)])
+@pytest.mark.parametrize("M,K,N,p", [(100, 80, 64, 0.1), (300, 500, 1500, 0.05), (17, 33, 3, 0.5),
+                                     # 64 x 64 tiles staged in LDS (N % 64 == 0): ragged edges,
+                                     # several rounds of 512 bits, wave-per-bit sparse tiles
+                                     (1000, 700, 256, 0.1), (300, 500, 1536, 0.05),
+                                     (130, 200, 128, 0.6), (64, 64, 64, 1.0), (500, 400, 192, 0.003),
+                                     (65, 129, 64, 0.3)])
 def test_sddmm_fp32(oracle, bws, dev, M, K, N, p):
     A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=44)
     dC = bws.gen_f32(45, (M, N), "normal", 0.0, 1.0)
@@ -254,6 +259,39 @@ def test_sddmm_fp32(oracle, bws, dev, M, K, N, p):
     R = oracle.sddmm_f64(W, K, rp, dC, B)
     S = oracle.sddmm_f64(W, K, rp, dC, B, absolute=True)
     assert np.max(np.abs(dA - R) / np.maximum(S, 1e-30)) <= 1e-5
+
+
+@pytest.mark.parametrize("M,K,N,p", [(700, 900, 320, 0.08), (200, 300, 64, 0.7)])
+def test_sddmm_fp32_two_forms(oracle, bws, dev, M, K, N, p):
+    """FP32 SDDMM with the workspace (64 x 64 tiles with >= 64 set bits staged
+    in LDS, the rest by a wave per row) and without it (every row by a wave):
+    both within 1e-5 of the FP64 oracle relative to Σ|dC|·|B|, on a mask
+    mixing dense, sparse (< 64 bits) and empty tiles."""
+    import ctypes as C
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=M + K)
+    W = W.copy()
+    W[M // 3: M // 2] = 0                      # a band of empty tiles
+    lo, hi = M // 2, M // 2 + 70               # a band of sparse tiles
+    W[lo:hi] = bws.gen_bitmap_iid(7, hi - lo, K, 0.004)[:, :W.shape[1]]
+    A = bws.Bitmap(to_dev(W, dev), M, K)
+    rp = bws.row_ptr(A)
+    A = bws.Bitmap(A.words, M, K, rp)
+    rph = rp.cpu().numpy()
+    nnz = int(rph[-1])
+    dC = bws.gen_f32(46, (M, N), "normal", 0.0, 1.0)
+    dCd = to_dev(dC, dev)
+    R = oracle.sddmm_f64(W, K, rph, dC, B)
+    S = oracle.sddmm_f64(W, K, rph, dC, B, absolute=True)
+    s = A.struct()
+    lib = bws.lib()
+    wsz = lib.bws_sddmm_workspace_size(C.byref(s))
+    for ws in (None, bws._workspace(wsz, dev)):
+        out = torch.full((nnz,), float("nan"), device=dev)
+        st = lib.bws_sddmm(C.byref(s), bws._ptr(dCd), bws._ptr(Bd), 0, N, N, N, bws._ptr(out), 0,
+                           bws._ptr(ws), 0 if ws is None else ws.numel(), bws._stream(dev))
+        assert st == 0
+        got = out.cpu().numpy()
+        assert np.max(np.abs(got - R) / np.maximum(S, 1e-30)) <= 1e-5
 
 
 def test_spmm_batched(oracle, bws, dev):
