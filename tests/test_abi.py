@@ -144,6 +144,25 @@ def test_sddmm_and_groups_validation(bws):
     assert L.bws_batch_prepare(arr, 1, 0, 0, None, FAKE, ws, C.byref(b)) == bws.BWS_EINVAL
 
 
+def test_workspace_sizes(bws):
+    """Workspace queries: BF16X2 on a 0/1 A asks for B's two BF16 planes (when
+    N % 8 == 0 and the plan allows MFMA), nothing otherwise; the SDDMM
+    workspace holds the per-word offsets and one byte per 64 x 64 tile."""
+    L = bws.lib()
+    P = bws.PRECISION
+    a = _bm(300, 200, 7)
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16x2"], None) >= 2 * 200 * 64 * 2
+    assert L.bws_spmm_workspace_size(C.byref(a), 60, P["bf16x2"], None) == 0
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16"], None) == 0
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp32"], None) == 0
+    plan = bws.make_plan(kernel=1)
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16x2"], C.byref(plan)) == 0
+    av = _bm(300, 200, 7, row_ptr=FAKE, values=FAKE)
+    assert L.bws_spmm_workspace_size(C.byref(av), 64, P["bf16x2"], None) == 0
+    assert L.bws_spmm_workspace_size(C.byref(av), 64, P["bf16"], None) >= 300 * 256 * 2
+    assert L.bws_sddmm_workspace_size(C.byref(a)) >= 300 * 7 * 4 + 5 * 4
+
+
 def test_transpose_perm_gather_validation(bws):
     L = bws.lib()
     a = _bm(40, 64, 2, row_ptr=FAKE)
