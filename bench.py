@@ -25,6 +25,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFS = 2500.0  # dense BF16 / FP16
+# FP32 vector adds per second: 157.3 TFLOP/s of FP32 vector FMA counts packed
+# (two-wide) FMAs as 2 flops each; this toolchain builds with packed FP32 off,
+# so the rate it can reach is half of the packed one.
+ADD_RATE_PACKED = 157.3e12 / 2
+ADD_RATE_UNPACKED = ADD_RATE_PACKED / 2
 # HBM bytes per launch of the benchmark kernel, from the FETCH_SIZE / WRITE_SIZE
 # counter runs summarised in profiles/ (one file per kernel)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "spmm_{}_4096_clustered_traffic.json")
@@ -198,11 +203,20 @@ def main():
             traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
+    t_hbm_us = alg_bytes / (HBM_PEAK_GBS * 1e9) * 1e6
     if args.precision == "fp32":
+        # binding ceiling of the vector-add path: max(HBM time, nnz*N adds)
+        t_add_us = nnz * N / ADD_RATE_PACKED * 1e6
+        t_add_unpacked_us = nnz * N / ADD_RATE_UNPACKED * 1e6
         roofline = {"kernel": "k_spmm_cluster" if path == "fp32_rowclusters" else "k_spmm_rows2",
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel_ms": round(ms_kernel, 5), "algorithmic_bytes": alg_bytes}
+                    "kernel_ms": round(ms_kernel, 5), "algorithmic_bytes": alg_bytes,
+                    "ceiling": {"hbm_us": round(t_hbm_us, 2), "fp32_add_us": round(t_add_us, 2),
+                                "fp32_add_unpacked_us": round(t_add_unpacked_us, 2),
+                                "binding": "fp32_add" if t_add_us > t_hbm_us else "hbm",
+                                "frac_of_binding": round(max(t_add_us, t_hbm_us) / (ms_kernel * 1e3), 4),
+                                "frac_of_unpacked": round(max(t_add_unpacked_us, t_hbm_us) / (ms_kernel * 1e3), 4)}}
     else:
         # the MFMA kernels run every 32-wide K step of a stage that has a set
         # bit: at these densities that is the dense product (2*M*N*K per pass,
@@ -210,10 +224,14 @@ def main():
         passes = 2 if args.precision == "bf16x2" else 1
         mfma_flops = 2.0 * M * N * K * passes
         tf = mfma_flops / (ms_kernel * 1e-3) / 1e12
+        t_mfma_us = mfma_flops / (MFMA_PEAK_TFS * 1e12) * 1e6
         roofline = {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_PEAK_TFS,
                     "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFS, 4), "traffic": traffic,
                     "kernel_ms": round(ms_kernel, 5), "mfma_flops": mfma_flops,
-                    "hbm_achieved_GBps": round(achieved, 1), "algorithmic_bytes": alg_bytes}
+                    "hbm_achieved_GBps": round(achieved, 1), "algorithmic_bytes": alg_bytes,
+                    "ceiling": {"hbm_us": round(t_hbm_us, 2), "mfma_us": round(t_mfma_us, 2),
+                                "binding": "mfma" if t_mfma_us > t_hbm_us else "hbm",
+                                "frac_of_binding": round(max(t_mfma_us, t_hbm_us) / (ms_kernel * 1e3), 4)}}
 
     extras = {}
     if not args.no_extras:
