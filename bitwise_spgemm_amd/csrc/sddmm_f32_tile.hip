@@ -4,16 +4,20 @@
 // A workgroup takes one tile (64 rows x 64 columns of A, two bitmap words per
 // row).  Both operands' rows for the tile (dC rows r0.., B rows k0..) are
 // copied into LDS 64 values of N at a time by global_load_lds_dwordx4 (two
-// stages in flight, 16 KiB per operand per stage), and each thread computes
-// whole dot products for up to two of the tile's set bits per round, reading
-// 16-byte chunks of both rows from LDS: B's rows are read from L2 once per
-// tile instead of once per set bit, and no cross-lane reduction is needed.
-// Chunk c of staged row r sits at slot c ^ (r & 15), so lanes reading the
-// same chunk of different rows spread over the 16 bank groups.
+// stages in flight, 16 KiB per operand per stage).  Each set bit's dot
+// product belongs to a quad of lanes: lane p of the quad reads 16-byte chunks
+// 4i + p (i = 0..3) of both rows from LDS, so B's rows are read from L2 once
+// per tile instead of once per set bit.  Chunk c of staged row r sits at slot
+// 4 ((c >> 2) ^ (r & 3)) + ((c & 3) ^ ((r >> 2) & 3)): a quad fills one
+// aligned group of four slots, consecutive rows (the dC rows of consecutive
+// bits) fall in different groups, and the four quads of a 16-lane LDS pass
+// meet in a group only when their B rows agree in bits 0-1 (~2-way conflicts,
+// against ~3-way when every lane reads a whole row).
 //
-// Each dot product runs over N in ascending stages of 64 with four partial
-// sums (components x, y, z, w of the 16-byte chunks), added as
-// (x + y) + (z + w) at the end: a fixed order, so results are reproducible.
+// Each lane keeps four partial sums (components x, y, z, w of its chunks) over
+// N in ascending stages of 64; at the end they are added as (x + y) + (z + w)
+// and the quad's four as (p0 + p1) + (p2 + p3): a fixed order, so results are
+// reproducible.
 // Tiles with fewer than 64 set bits are only flagged (dense_tile[tile] = 0):
 // staging 2·64 rows for so few dot products costs more than the wave-per-row
 // kernel, which then takes exactly those tiles' bits.
@@ -32,10 +36,13 @@ constexpr int NS = 64;                // N values per stage
 constexpr int ROWB = NS * 4;          // bytes per staged row (16 chunks)
 constexpr int OPB = TT * ROWB;        // bytes per operand per stage
 constexpr int NCW = OPB / 1024 / 4;   // 1 KiB copies per wave per operand per stage
-constexpr int SLOTS = 2;              // set bits per thread per round
+constexpr int SLOTS = 8;              // set bits per quad of lanes per round
+constexpr int ROUND = SLOTS * 64;     // set bits per round (64 quads)
 constexpr int SPARSE = 64;            // tiles with fewer set bits are left to the row kernel
 
-__device__ __forceinline__ uint32_t chunk_slot(int r, int c) { return (uint32_t)(c ^ (r & 15)); }
+__device__ __forceinline__ uint32_t chunk_slot(int r, int c) {
+  return (uint32_t)(4 * ((c >> 2) ^ (r & 3)) + ((c & 3) ^ ((r >> 2) & 3)));
+}
 
 __device__ __forceinline__ void fma4(float4& acc, const float4& a, const float4& b) {
   acc.x = fmaf(a.x, b.x, acc.x);
@@ -105,7 +112,8 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   };
 
   // copy sources: instruction g of a stage fills staged rows 4g .. 4g+3; lane
-  // l writes slot l % 16 of row 4g + l / 16, i.e. chunk (l % 16) ^ (row & 15)
+  // l writes slot l % 16 of row 4g + l / 16, which holds chunk
+  // chunk_slot(row, l % 16) (the mapping is its own inverse)
   const float* dsrc[NCW];
   const float* bsrc[NCW];
 #pragma unroll
@@ -130,50 +138,67 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   };
   const int64_t nst = N / NS;
 
-  for (int q = 0; q < total; q += SLOTS * 256) {
-    int rr[SLOTS], kk[SLOTS];
+  const int ph = lane & 3;                  // this lane's place in its quad
+  const int quad = wave * 16 + (lane >> 2);  // 0 .. 63
+  const uint8_t* sDp = &sD[0][0];
+  const uint8_t* sBp = &sB[0][0];
+  for (int q = 0; q < total; q += ROUND) {
+    // Per slot: the byte offsets of this lane's reads.  With x = r & 3 and
+    // y = (r >> 2) & 3, chunk 4i + ph of row r sits at slot 4 (i ^ x) + (ph ^ y):
+    // reading the dC row's slot groups g = 0..3 in order takes its chunks
+    // i = g ^ x_d, and the matching B chunk is in group g ^ x_d ^ x_b.
+    uint32_t offD[SLOTS], offB[SLOTS][4];
     int64_t out[SLOTS];
     bool ok[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
-      const int j = q + 256 * s + tid;
+      const int j = q + 64 * s + quad;
       ok[s] = j < total;
-      rr[s] = 0;
-      kk[s] = 0;
+      int rr = 0, kk = 0;
       out[s] = 0;
-      if (ok[s]) locate(j, rr[s], kk[s], out[s]);
+      if (ok[s]) locate(j, rr, kk, out[s]);
+      offD[s] = (uint32_t)(rr * ROWB + 16 * (ph ^ ((rr >> 2) & 3)));
+      const int z = (rr ^ kk) & 3;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        offB[s][g] = (uint32_t)(kk * ROWB + 16 * (ph ^ ((kk >> 2) & 3)) + 64 * (g ^ z));
     }
-    // slot 1 is used by this wave at all (uniform)
-    const bool two = q + 256 + 64 * wave < total;
     float4 acc[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // slots used by some quad of this wave (uniform)
+    const int left = total - q - 16 * wave;
+    const int used = left <= 0 ? 0 : (left + 63) / 64 < SLOTS ? (left + 63) / 64 : SLOTS;
 
-    issue(0);
-    for (int64_t st = 0; st < nst; ++st) {
+    auto stage = [&](int64_t st, uint32_t buf) {
       wait_vmcnt<0>();  // this wave's copies of stage st landed
       __syncthreads();  // everyone's; stage st-1's buffer is free
       if (st + 1 < nst) issue(st + 1);
-      const uint8_t* bufD = &sD[0][0] + (uint32_t)(st & 1) * OPB;
-      const uint8_t* bufB = &sB[0][0] + (uint32_t)(st & 1) * OPB;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        const float4 a = *reinterpret_cast<const float4*>(bufD + rr[0] * ROWB + 16 * chunk_slot(rr[0], c));
-        const float4 b = *reinterpret_cast<const float4*>(bufB + kk[0] * ROWB + 16 * chunk_slot(kk[0], c));
-        fma4(acc[0], a, b);
-      }
-      if (two) {
+      for (int s = 0; s < SLOTS; ++s) {
+        if (s < used) {  // some quad of this wave has slot s
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-          const float4 a = *reinterpret_cast<const float4*>(bufD + rr[1] * ROWB + 16 * chunk_slot(rr[1], c));
-          const float4 b = *reinterpret_cast<const float4*>(bufB + kk[1] * ROWB + 16 * chunk_slot(kk[1], c));
-          fma4(acc[1], a, b);
+          for (int g = 0; g < 4; ++g) {
+            const float4 a = *reinterpret_cast<const float4*>(sDp + buf + offD[s] + 64 * g);
+            const float4 b = *reinterpret_cast<const float4*>(sBp + buf + offB[s][g]);
+            fma4(acc[s], a, b);
+          }
         }
       }
+    };
+    issue(0);
+    for (int64_t st = 0; st < nst; st += 2) {
+      stage(st, 0u);
+      if (st + 1 < nst) stage(st + 1, (uint32_t)OPB);
     }
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s)
-      if (ok[s]) dA[out[s]] = (acc[s].x + acc[s].y) + (acc[s].z + acc[s].w);
+    for (int s = 0; s < SLOTS; ++s) {
+      if (s >= used) break;
+      const float v = (acc[s].x + acc[s].y) + (acc[s].z + acc[s].w);
+      const float v2 = v + __shfl_xor(v, 1, kWave);     // (p0 + p1), (p2 + p3)
+      const float v4 = v2 + __shfl_xor(v2, 2, kWave);   // (p0 + p1) + (p2 + p3)
+      if (ph == 0 && ok[s]) dA[out[s]] = v4;
+    }
     __syncthreads();  // the buffers are reused by the next round
   }
 }
