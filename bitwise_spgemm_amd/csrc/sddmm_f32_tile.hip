@@ -4,20 +4,21 @@
 // A workgroup takes one tile (64 rows x 64 columns of A, two bitmap words per
 // row).  Both operands' rows for the tile (dC rows r0.., B rows k0..) are
 // copied into LDS 64 values of N at a time by global_load_lds_dwordx4 (two
-// stages in flight, 16 KiB per operand per stage).  Each set bit's dot
-// product belongs to a quad of lanes: lane p of the quad reads 16-byte chunks
-// 4i + p (i = 0..3) of both rows from LDS, so B's rows are read from L2 once
-// per tile instead of once per set bit.  Chunk c of staged row r sits at slot
+// stages in flight, 16 KiB per operand per stage), so B's rows are read from
+// L2 once per tile instead of once per set bit.  A work item is two
+// consecutive set bits of one row (a row with an odd count ends with a single
+// bit) and belongs to a quad of lanes: lane p of the quad reads 16-byte chunks
+// 4i + p (i = 0..3) of the dC row once and of both B rows, i.e. 12 LDS reads
+// for two dot products instead of 16.  Chunk c of staged row r sits at slot
 // 4 ((c >> 2) ^ (r & 3)) + ((c & 3) ^ ((r >> 2) & 3)): a quad fills one
-// aligned group of four slots, consecutive rows (the dC rows of consecutive
-// bits) fall in different groups, and the four quads of a 16-lane LDS pass
-// meet in a group only when their B rows agree in bits 0-1 (~2-way conflicts,
-// against ~3-way when every lane reads a whole row).
+// aligned group of four slots, the dC reads of a lane take fixed offsets from
+// one base (its four groups in order), and the four quads of a 16-lane LDS
+// pass meet in a group only when their rows agree in bits 0-1.
 //
-// Each lane keeps four partial sums (components x, y, z, w of its chunks) over
-// N in ascending stages of 64; at the end they are added as (x + y) + (z + w)
-// and the quad's four as (p0 + p1) + (p2 + p3): a fixed order, so results are
-// reproducible.
+// Each lane keeps four partial sums per bit (components x, y, z, w of its
+// chunks) over N in ascending stages of 64; at the end they are added as
+// (x + y) + (z + w) and the quad's four as (p0 + p1) + (p2 + p3): a fixed
+// order, so results are reproducible.
 // Tiles with fewer than 64 set bits are only flagged (dense_tile[tile] = 0):
 // staging 2·64 rows for so few dot products costs more than the wave-per-row
 // kernel, which then takes exactly those tiles' bits.
@@ -31,13 +32,12 @@ namespace bws {
 namespace {
 
 constexpr int TT = 64;                // tile: TT rows x TT columns of A
-constexpr int TW = 2 * TT;            // bitmap words per tile
 constexpr int NS = 64;                // N values per stage
 constexpr int ROWB = NS * 4;          // bytes per staged row (16 chunks)
 constexpr int OPB = TT * ROWB;        // bytes per operand per stage
 constexpr int NCW = OPB / 1024 / 4;   // 1 KiB copies per wave per operand per stage
-constexpr int SLOTS = 8;              // set bits per quad of lanes per round
-constexpr int ROUND = SLOTS * 64;     // set bits per round (64 quads)
+constexpr int G = 2;                  // set bits of one row per work item
+constexpr int SLOTS = 8;              // work items per quad of lanes per round
 constexpr int SPARSE = 64;            // tiles with fewer set bits are left to the row kernel
 
 __device__ __forceinline__ uint32_t chunk_slot(int r, int c) {
@@ -51,66 +51,19 @@ __device__ __forceinline__ void fma4(float4& acc, const float4& a, const float4&
   acc.w = fmaf(a.w, b.w, acc.w);
 }
 
-__global__ void __launch_bounds__(256, 2)
-k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-                 const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ woff,
-                 const float* __restrict__ dC, int64_t lddc, const float* __restrict__ B,
-                 int64_t ldb, int64_t N, float* __restrict__ dA, uint8_t* __restrict__ dense_tile,
-                 int64_t row_blocks, int64_t k_blocks, int64_t per_xcd) {
-  __shared__ __attribute__((aligned(1024))) uint8_t sD[2][OPB];
-  __shared__ __attribute__((aligned(1024))) uint8_t sB[2][OPB];
-  __shared__ int64_t sBase[TW];   // value index of each tile word's first bit
-  __shared__ uint32_t sW[TW];     // the tile's words, row-major (two per row)
-  __shared__ int32_t sPre[TW + 1];  // exclusive prefix of their popcounts
+// The tile's rounds: work item i is set bits G p .. G p + G-1 (in column
+// order) of one row; SLOTS items per quad per round.
+__device__ __forceinline__ void tile_rounds(const float* __restrict__ dC, int64_t lddc,
+                                            const float* __restrict__ B, int64_t ldb, int64_t N,
+                                            float* __restrict__ dA, int64_t M, int64_t K,
+                                            int64_t r0, int64_t k0, int n_items,
+                                            const int32_t* sItem, const uint64_t* sMask,
+                                            const int64_t* sBase, uint8_t (*sD)[OPB],
+                                            uint8_t (*sB)[OPB]) {
+  constexpr int ROUND = SLOTS * 64;  // work items per round (64 quads)
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
-  if (tile >= row_blocks * k_blocks) return;
-  // row blocks vary fastest: the tiles an XCD runs together share B's rows
-  const int64_t kb = tile / row_blocks, rb = tile - kb * row_blocks;
-  const int64_t r0 = rb * TT, k0 = kb * TT;
-  const int64_t wpr = words_for(K);
-
-  int cnt = 0;
-  const int64_t wr = r0 + (tid >> 1), ww = k0 / 32 + (tid & 1);  // thread tid < TW: word tid
-  const bool in = tid < TW && wr < M && ww < wpr;
-  if (tid < TW) {
-    const uint32_t word = in ? W[wr * ldw + ww] & valid_bits(ww, K) : 0u;
-    cnt = __popc(word);
-    sW[tid] = word;
-  }
-  // exclusive prefix over the 128 words: a scan in each of waves 0 and 1,
-  // then wave 1 adds wave 0's total
-  int incl = 0;
-  if (wave < 2) incl = wave_incl_scan(cnt);
-  if (tid < TW) sPre[tid + 1] = incl;
-  if (tid == 0) sPre[0] = 0;
-  __syncthreads();
-  if (tid >= TT && tid < TW) sPre[tid + 1] = incl + sPre[TT];
-  __syncthreads();
-  const int total = sPre[TW];
-  if (tid == 0) dense_tile[tile] = total >= SPARSE ? 1 : 0;
-  if (total < SPARSE) return;
-  if (tid < TW) sBase[tid] = in ? row_ptr[wr] + woff[wr * wpr + ww] : 0;
-  __syncthreads();
-
-  // set bit j of the tile (row-major, MSB-first within a word): staged rows
-  // rr (of dC) and kk (of B) and its value index
-  auto locate = [&](int j, int& rr, int& kk, int64_t& out) {
-    int lo = 0, hi = TW;  // sPre[lo] <= j < sPre[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sPre[mid] <= j) lo = mid;
-      else hi = mid;
-    }
-    uint32_t m = sW[lo];
-    for (int x = sPre[lo]; x < j; ++x) m &= ~wbit(__clz((int)m));  // drop earlier bits
-    rr = lo >> 1;
-    kk = 32 * (lo & 1) + __clz((int)m);
-    out = sBase[lo] + (j - sPre[lo]);
-  };
-
   // copy sources: instruction g of a stage fills staged rows 4g .. 4g+3; lane
   // l writes slot l % 16 of row 4g + l / 16, which holds chunk
   // chunk_slot(row, l % 16) (the mapping is its own inverse)
@@ -142,32 +95,58 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   const int quad = wave * 16 + (lane >> 2);  // 0 .. 63
   const uint8_t* sDp = &sD[0][0];
   const uint8_t* sBp = &sB[0][0];
-  for (int q = 0; q < total; q += ROUND) {
-    // Per slot: the byte offsets of this lane's reads.  With x = r & 3 and
-    // y = (r >> 2) & 3, chunk 4i + ph of row r sits at slot 4 (i ^ x) + (ph ^ y):
-    // reading the dC row's slot groups g = 0..3 in order takes its chunks
-    // i = g ^ x_d, and the matching B chunk is in group g ^ x_d ^ x_b.
-    uint32_t offD[SLOTS], offB[SLOTS][4];
+  for (int q = 0; q < n_items; q += ROUND) {
+    // Item i of the tile: set bits G p .. G p + G-1 (in column order) of one
+    // row.  Per item: the dC row's read offsets and, per bit, the B row's.
+    // With x = r & 3 and y = (r >> 2) & 3, chunk 4i + ph of staged row r sits
+    // at slot 4 (i ^ x) + (ph ^ y): reading the dC row's slot groups g = 0..3
+    // in order takes its chunks i = g ^ x_d, whose B chunks are in groups
+    // g ^ x_d ^ x_b.
+    uint32_t offD[SLOTS], offB[SLOTS][G][4];
     int64_t out[SLOTS];
-    bool ok[SLOTS];
+    int nb[SLOTS];  // set bits of the item (0 .. G)
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
-      const int j = q + 64 * s + quad;
-      ok[s] = j < total;
-      int rr = 0, kk = 0;
+      const int i = q + 64 * s + quad;
+      int rr = 0;
+      nb[s] = 0;
       out[s] = 0;
-      if (ok[s]) locate(j, rr, kk, out[s]);
+      uint64_t m = 0;
+      int first = 0;
+      if (i < n_items) {
+        int lo = 0, hi = TT;  // sItem[lo] <= i < sItem[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (sItem[mid] <= i) lo = mid;
+          else hi = mid;
+        }
+        rr = lo;
+        m = sMask[rr];
+        first = G * (i - sItem[rr]);
+        const int c = __popcll(m) - first;
+        nb[s] = c < G ? c : G;
+        out[s] = sBase[rr] + first;
+        for (int x = 0; x < first; ++x) m &= ~(0x8000000000000000ull >> __clzll(m));
+      }
       offD[s] = (uint32_t)(rr * ROWB + 16 * (ph ^ ((rr >> 2) & 3)));
-      const int z = (rr ^ kk) & 3;
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
-        offB[s][g] = (uint32_t)(kk * ROWB + 16 * (ph ^ ((kk >> 2) & 3)) + 64 * (g ^ z));
+      for (int b = 0; b < G; ++b) {
+        // column of the item's bit b (row 0 when the item has fewer bits)
+        const int kk = b < nb[s] ? __clzll(m) : 0;
+        if (b < nb[s]) m &= ~(0x8000000000000000ull >> kk);
+        const int z = (rr ^ kk) & 3;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          offB[s][b][g] = (uint32_t)(kk * ROWB + 16 * (ph ^ ((kk >> 2) & 3)) + 64 * (g ^ z));
+      }
     }
-    float4 acc[SLOTS];
+    float4 acc[SLOTS][G];
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = 0; s < SLOTS; ++s)
+#pragma unroll
+      for (int b = 0; b < G; ++b) acc[s][b] = make_float4(0.f, 0.f, 0.f, 0.f);
     // slots used by some quad of this wave (uniform)
-    const int left = total - q - 16 * wave;
+    const int left = n_items - q - 16 * wave;
     const int used = left <= 0 ? 0 : (left + 63) / 64 < SLOTS ? (left + 63) / 64 : SLOTS;
 
     auto stage = [&](int64_t st, uint32_t buf) {
@@ -180,8 +159,11 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
             const float4 a = *reinterpret_cast<const float4*>(sDp + buf + offD[s] + 64 * g);
-            const float4 b = *reinterpret_cast<const float4*>(sBp + buf + offB[s][g]);
-            fma4(acc[s], a, b);
+#pragma unroll
+            for (int b = 0; b < G; ++b) {
+              const float4 v = *reinterpret_cast<const float4*>(sBp + buf + offB[s][b][g]);
+              fma4(acc[s][b], a, v);
+            }
           }
         }
       }
@@ -194,13 +176,67 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
       if (s >= used) break;
-      const float v = (acc[s].x + acc[s].y) + (acc[s].z + acc[s].w);
-      const float v2 = v + __shfl_xor(v, 1, kWave);     // (p0 + p1), (p2 + p3)
-      const float v4 = v2 + __shfl_xor(v2, 2, kWave);   // (p0 + p1) + (p2 + p3)
-      if (ph == 0 && ok[s]) dA[out[s]] = v4;
+#pragma unroll
+      for (int b = 0; b < G; ++b) {
+        const float v = (acc[s][b].x + acc[s][b].y) + (acc[s][b].z + acc[s][b].w);
+        const float v2 = v + __shfl_xor(v, 1, kWave);    // (p0 + p1), (p2 + p3)
+        const float v4 = v2 + __shfl_xor(v2, 2, kWave);  // (p0 + p1) + (p2 + p3)
+        if (ph == 0 && b < nb[s]) dA[out[s] + b] = v4;
+      }
     }
     __syncthreads();  // the buffers are reused by the next round
   }
+}
+
+__global__ void __launch_bounds__(256, 2)
+k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+                 const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ woff,
+                 const float* __restrict__ dC, int64_t lddc, const float* __restrict__ B,
+                 int64_t ldb, int64_t N, float* __restrict__ dA, uint8_t* __restrict__ dense_tile,
+                 int64_t row_blocks, int64_t k_blocks, int64_t per_xcd) {
+  __shared__ __attribute__((aligned(1024))) uint8_t sD[2][OPB];
+  __shared__ __attribute__((aligned(1024))) uint8_t sB[2][OPB];
+  __shared__ int64_t sBase[TT];      // value index of each row's first bit in the tile
+  __shared__ uint64_t sMask[TT];     // each row's 64 tile columns, MSB = first column
+  __shared__ int32_t sItem[TT + 1];  // exclusive prefix of the rows' item counts
+  __shared__ int32_t sTotal;         // set bits in the tile
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * k_blocks) return;
+  // row blocks vary fastest: the tiles an XCD runs together share B's rows
+  const int64_t kb = tile / row_blocks, rb = tile - kb * row_blocks;
+  const int64_t r0 = rb * TT, k0 = kb * TT;
+  const int64_t wpr = words_for(K);
+
+  // wave 0: lane l takes row r0 + l (its two words, its bit count, its items)
+  const int64_t wr = r0 + lane, w0 = k0 / 32;
+  const bool row_in = wave == 0 && wr < M;
+  uint64_t mask = 0;
+  if (row_in) {
+    const uint32_t a = w0 < wpr ? W[wr * ldw + w0] & valid_bits(w0, K) : 0u;
+    const uint32_t b = w0 + 1 < wpr ? W[wr * ldw + w0 + 1] & valid_bits(w0 + 1, K) : 0u;
+    mask = ((uint64_t)a << 32) | b;
+  }
+  const int cnt = __popcll(mask);
+  if (wave == 0) {
+    const int bits = wave_sum(cnt);
+    const int incl = wave_incl_scan((cnt + G - 1) / G);
+    sItem[lane + 1] = incl;
+    if (lane == 0) sItem[0] = 0;
+    sMask[lane] = mask;
+    if (lane == 0) sTotal = bits;
+  }
+  __syncthreads();
+  const int total = sTotal;
+  if (tid == 0) dense_tile[tile] = total >= SPARSE ? 1 : 0;
+  if (total < SPARSE) return;
+  if (row_in && cnt > 0) sBase[lane] = row_ptr[wr] + woff[wr * wpr + w0];
+  const int n_items = sItem[TT];
+  __syncthreads();
+
+  tile_rounds(dC, lddc, B, ldb, N, dA, M, K, r0, k0, n_items, sItem, sMask, sBase, sD, sB);
 }
 
 }  // namespace
