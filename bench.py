@@ -292,6 +292,18 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     def tl(f, k=20, w=3):
         spin_up(f, 0.1)   # steady clocks before every comparison
         return round(time_loop(f, k, w)[1], 4)
+    # the device and its sustained HBM rate, beside the published 8 TB/s peak:
+    # a 1 GiB device copy (2 GiB moved) and a 1 GiB fill
+    props = torch.cuda.get_device_properties(dev)
+    big = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    big2 = torch.empty_like(big)
+    copy_ms = tl(lambda: big2.copy_(big), 10, 2)
+    fill_ms = tl(lambda: big.fill_(1.0), 10, 2)
+    ex["platform"] = {"name": props.name, "arch": getattr(props, "gcnArchName", ""),
+                      "cus": props.multi_processor_count,
+                      "hbm_copy_TBps": round(2 * (1 << 30) / (copy_ms * 1e-3) / 1e12, 2),
+                      "hbm_fill_TBps": round((1 << 30) / (fill_ms * 1e-3) / 1e12, 2)}
+    del big, big2
     Ad = bws.unpack_bitmap(A)
     ex["pack_ms"] = tl(lambda: bws.pack_bitmap(Ad), 5, 2)
     del Ad
