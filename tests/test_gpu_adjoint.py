@@ -60,12 +60,17 @@ def test_sddmm_mfma_kernels_agree(bws, dev, M, K, N):
         assert torch.equal(x, y), prec
 
 
-def test_sddmm_fp32_strided(oracle, bws, dev):
-    M, K, N = 200, 300, 130
+@pytest.mark.parametrize("N,pad_c,pad_b", [(130, 3, 0),    # wave per row
+                                            (128, 4, 8),    # 64 x 64 tiles, strided dC and B
+                                            (192, 1, 0)])   # ldc not a multiple of 4: wave per row
+def test_sddmm_fp32_strided(oracle, bws, dev, N, pad_c, pad_b):
+    M, K = 200, 300
     A, W, rp, _, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=3)
-    dC = bws.gen_f32(8, (M, N + 3), "normal", 0.0, 1.0)
+    dC = bws.gen_f32(8, (M, N + pad_c), "normal", 0.0, 1.0)
+    Bp = np.zeros((K, N + pad_b), np.float32)
+    Bp[:, :N] = B
     A = bws.Bitmap(A.words, M, K, to_dev(rp, dev))
-    got = bws.sddmm(A, to_dev(dC, dev)[:, :N], Bd, "fp32").cpu().numpy()
+    got = bws.sddmm(A, to_dev(dC, dev)[:, :N], to_dev(Bp, dev)[:, :N], "fp32").cpu().numpy()
     assert sddmm_err(oracle, W, K, rp, np.ascontiguousarray(dC[:, :N]), B, got) <= 1e-5
 
 

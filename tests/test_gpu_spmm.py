@@ -154,6 +154,25 @@ def test_spmm_bf16x2_two_forms(oracle, bws, dev, M, K, N, p, beta):
     assert d.max() <= 2e-6, d.max()
 
 
+@pytest.mark.parametrize("b_off", [4, 1])  # 16-byte aligned B rows (vector split) or not
+def test_spmm_bf16x2_strided_alpha_beta(oracle, bws, dev, rng, b_off):
+    """Split BF16 through the workspace with strided B and C, alpha and beta."""
+    M, K, N = 300, 260, 136
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.1, seed=17)
+    Bbig = np.zeros((K, N + 12), np.float32)
+    Bbig[:, b_off:b_off + N] = B
+    Bv = to_dev(Bbig, dev)[:, b_off:b_off + N]
+    C0 = rng.standard_normal((M, N + 8)).astype(np.float32)
+    Cbig = to_dev(C0, dev)
+    out = Cbig[:, :N]
+    bws.spmm(A, Bv, "bf16x2", alpha=0.5, beta=2.0, out=out)
+    got = Cbig.cpu().numpy()
+    ref = 0.5 * oracle.spmm_f64(W, K, B) + 2.0 * C0[:, :N].astype(np.float64)
+    S = 0.5 * oracle.spmm_f64(W, K, np.abs(B)) + 2.0 * np.abs(C0[:, :N]).astype(np.float64)
+    assert np.max(np.abs(got[:, :N] - ref) / np.maximum(S, 1e-30)) <= 1e-5
+    assert np.array_equal(got[:, N:], C0[:, N:])  # the padding columns are untouched
+
+
 TILES = {"bf16": [(128, 128), (128, 256), (256, 128), (256, 256)],
          "fp16": [(128, 128), (128, 256), (256, 128), (256, 256)],
          "bf16x2": [(128, 128), (256, 128)]}
