@@ -10,8 +10,10 @@
 // per wave).  Per 32-wide K step a wave reads 8 + 8 fragments for 64 MFMAs
 // (the 128 x 128-block kernel: 8 + 8 for 32).  Each output's N steps run in
 // ascending order, 32 at a time, as in the other MFMA SDDMM kernels, so all of
-// them give the same result.  (32-value stages with four in flight measured
-// 5% slower.)
+// them give the same result.  A stage's 32 fragment reads are all issued
+// before its first MFMA (3% faster than reading each row of tiles' A fragment
+// just before its MFMAs); 32-value stages with four in flight measured 3-8%
+// slower.
 #include "mfma_common.hpp"
 
 namespace bws {
@@ -82,7 +84,7 @@ k_sddmm_mfma_big(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   }
   const uint32_t sC0 = lds_u32(&sC[0][0]), sB0 = lds_u32(&sB[0][0]);
   auto issue = [&](int64_t st) {
-    const uint32_t buf = (uint32_t)(st & (NBUF - 1)) * STAGE;
+    const uint32_t buf = (uint32_t)(st % NBUF) * STAGE;
 #pragma unroll
     for (int t = 0; t < NC_W; ++t) {
       const uint32_t off = buf + (uint32_t)((wave * NC_W + t) * 1024);
@@ -108,33 +110,40 @@ k_sddmm_mfma_big(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
 
   for (int64_t st = 0; st < nst; ++st) {
     // stage st has landed when at most the later stages' copies are out
-    if constexpr (NBUF == 4) {
-      if (st + 2 < nst) wait_vmcnt<2 * 2 * NC_W>();
-      else if (st + 1 < nst) wait_vmcnt<2 * NC_W>();
+    // (up to NBUF - 2 stages were issued after it)
+    {
+      const int64_t later = nst - 1 - st < NBUF - 2 ? nst - 1 - st : NBUF - 2;
+      static_assert(NBUF >= 2 && NBUF <= 5, "two to five stages");
+      if (NBUF >= 5 && later >= 3) wait_vmcnt<(NBUF >= 5 ? 3 : 0) * 2 * NC_W>();
+      else if (NBUF >= 4 && later >= 2) wait_vmcnt<(NBUF >= 4 ? 2 : 0) * 2 * NC_W>();
+      else if (NBUF >= 3 && later >= 1) wait_vmcnt<(NBUF >= 3 ? 1 : 0) * 2 * NC_W>();
       else wait_vmcnt<0>();
-    } else {
-      static_assert(NBUF == 2, "two or four stages");
-      wait_vmcnt<0>();
     }
     __syncthreads();  // every wave's copies of stage st; stage st-1's buffer is free
     if (st + NBUF - 1 < nst) issue(st + NBUF - 1);
-    const uint8_t* bufC = &sC[0][0] + (uint32_t)(st & (NBUF - 1)) * STAGE;
-    const uint8_t* bufB = &sB[0][0] + (uint32_t)(st & (NBUF - 1)) * STAGE;
+    const uint8_t* bufC = &sC[0][0] + (uint32_t)(st % NBUF) * STAGE;
+    const uint8_t* bufB = &sB[0][0] + (uint32_t)(st % NBUF) * STAGE;
+    // every fragment of the stage is requested before its first MFMA: with
+    // one wave per SIMD nothing else hides an LDS read's latency, and left to
+    // itself the compiler reloads one A fragment register per row of tiles
+    uint4 af[KSTEPS][8], bf[KSTEPS][8];
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
       const uint32_t chunk = (uint32_t)(16 * slot_of(j, 4 * ks + q));
-      uint4 af[8], bf[8];
 #pragma unroll
       for (int a = 0; a < 8; ++a)
-        af[a] = *reinterpret_cast<const uint4*>(bufC + row_c + chunk + a * 16 * ROWB);
+        af[ks][a] = *reinterpret_cast<const uint4*>(bufC + row_c + chunk + a * 16 * ROWB);
 #pragma unroll
       for (int b = 0; b < 8; ++b)
-        bf[b] = *reinterpret_cast<const uint4*>(bufB + row_b + chunk + b * 16 * ROWB);
+        bf[ks][b] = *reinterpret_cast<const uint4*>(bufB + row_b + chunk + b * 16 * ROWB);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks)
 #pragma unroll
       for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = mfma<PREC>(af[a], bf[b], acc[a][b]);
-    }
+        for (int b = 0; b < 8; ++b) acc[a][b] = mfma<PREC>(af[ks][a], bf[ks][b], acc[a][b]);
   }
 
   // Epilogue.  The block's bitmap words and the value index of each word's
