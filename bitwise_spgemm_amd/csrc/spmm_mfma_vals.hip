@@ -141,25 +141,32 @@ __device__ __forceinline__ void vals_tile(const uint16_t* __restrict__ Ad, int64
     __syncthreads();
     if (st + 1 < nst) issue(st + 1);
     const uint32_t bbase = sB0 + (uint32_t)(buf * B_STAGE) + rdb;
+    // every fragment of the stage is requested before its first MFMA
+    constexpr int VKSTEPS = VKS / 32;
+    uint4 bf[VKSTEPS][VTN], af[VKSTEPS][VTM];
 #pragma unroll
-    for (int ks = 0; ks < VKS / 32; ++ks) {
-      uint4 bf[VTN];
+    for (int ks = 0; ks < VKSTEPS; ++ks) {
 #pragma unroll
       for (int b = 0; b < VTN; ++b) {
         const uint32_t a0 = bbase + (uint32_t)(32 * ((c0 + b) ^ fsw)) + (uint32_t)(32 * ks * B_ROWB);
         const uint2 lo = tr_read(a0);
         const uint2 hi = tr_read(a0 + 4 * B_ROWB);
-        bf[b] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        bf[ks][b] = make_uint4(lo.x, lo.y, hi.x, hi.y);
       }
 #pragma unroll
       for (int a = 0; a < VTM; ++a) {
         const int r = VWM * wm + 16 * a + j;
         const int slot = (4 * ks + q) ^ ((r >> 1) & 7);
-        const uint4 af = *reinterpret_cast<const uint4*>(&sA[buf][r * A_ROWB + 16 * slot]);
-#pragma unroll
-        for (int b = 0; b < VTN; ++b) acc[a][b] = mfma<PREC>(af, bf[b], acc[a][b]);
+        af[ks][a] = *reinterpret_cast<const uint4*>(&sA[buf][r * A_ROWB + 16 * slot]);
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < VKSTEPS; ++ks)
+#pragma unroll
+      for (int a = 0; a < VTM; ++a)
+#pragma unroll
+        for (int b = 0; b < VTN; ++b) acc[a][b] = mfma<PREC>(af[ks][a], bf[ks][b], acc[a][b]);
   }
 
 #pragma unroll
