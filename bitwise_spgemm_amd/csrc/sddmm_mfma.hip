@@ -322,26 +322,30 @@ k_sddmm_mfma_dma(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
     wait_vmcnt<0>();
     __syncthreads();
     if (st + 1 < nst) issue(st + 1);
+    // every fragment of the stage is requested before its first MFMA
+    uint4 af[NSTG / 32][4], bf[NSTG / 32][4];
 #pragma unroll
     for (int ks = 0; ks < NSTG / 32; ++ks) {
-      uint4 af[4], bf[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const int r = 64 * wm + 16 * a + j;
-        af[a] = *reinterpret_cast<const uint4*>(
+        af[ks][a] = *reinterpret_cast<const uint4*>(
             &sC[buf][r * ROWB_D + 16 * ((4 * ks + q) ^ ((r >> 1) & 7))]);
       }
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int r = 64 * wn + 16 * b + j;
-        bf[b] = *reinterpret_cast<const uint4*>(
+        bf[ks][b] = *reinterpret_cast<const uint4*>(
             &sB[buf][r * ROWB_D + 16 * ((4 * ks + q) ^ ((r >> 1) & 7))]);
       }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < NSTG / 32; ++ks)
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = mfma<PREC>(af[a], bf[b], acc[a][b]);
-    }
+        for (int b = 0; b < 4; ++b) acc[a][b] = mfma<PREC>(af[ks][a], bf[ks][b], acc[a][b]);
   }
 
   store_set_bits_128(acc, &sC[0][0], W, M, K, ldw, row_ptr, woff, dA, r0, k0, wm, wn, j, q);
