@@ -82,6 +82,37 @@ def _dense(oracle, W, K, rp, vals):
     return D, mask
 
 
+def test_sddmm_fp32_full_size(oracle, bws, dev):
+    """The FP32 SDDMM at the value-mask benchmark's shape (4096 x 4096 at 10%,
+    N = 2048; 64 x 64 tiles staged in LDS): every 16th row against the FP64
+    oracle, and linearity in dC over the whole output."""
+    M = K = 4096
+    N = 2048
+    W = bws.gen_bitmap_iid(21, M, K, 0.1)
+    A0 = bws.Bitmap(to_dev(W, dev), M, K)
+    rp_d = bws.row_ptr(A0)
+    A = bws.Bitmap(A0.words, M, K, rp_d)
+    rp = rp_d.cpu().numpy()
+    dC = torch.randn((M, N), device=dev)
+    B = torch.randn((K, N), device=dev)
+    got = bws.sddmm(A, dC, B, "fp32")
+    rows = np.arange(0, M, 16)
+    Ws = W[rows]
+    counts = rp[rows + 1] - rp[rows]
+    rps = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    dCs, Bh = dC.cpu().numpy()[rows], B.cpu().numpy()
+    R = oracle.sddmm_f64(Ws, K, rps, dCs, Bh)
+    S = oracle.sddmm_f64(Ws, K, rps, dCs, Bh, absolute=True)
+    g = got.cpu().numpy()
+    gs = np.concatenate([g[rp[r]:rp[r + 1]] for r in rows])
+    assert np.max(np.abs(gs - R) / np.maximum(S, 1e-30)) <= 1e-5
+    dC2 = torch.randn_like(dC)
+    lhs = bws.sddmm(A, dC + dC2, B, "fp32")
+    rhs = got + bws.sddmm(A, dC2, B, "fp32")
+    Sabs = bws.sddmm(A, dC.abs() + dC2.abs(), B.abs(), "fp32")
+    assert torch.max((lhs - rhs).abs() / Sabs.clamp_min(1e-30)).item() <= 3e-5
+
+
 def test_autograd_finite_differences(oracle, bws, dev):
     """C4's FD check: M = K = 256, N = 128, value-carrying mask, FP64 central
     differences of L = <G, C> computed on the CPU oracle."""
