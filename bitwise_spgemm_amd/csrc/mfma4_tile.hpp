@@ -190,6 +190,10 @@ __device__ __forceinline__ void mfma4_tile(const uint32_t* __restrict__ W, int64
     bool run = true;
     if constexpr (SKIP) run = __ballot(any != 0u) != 0ull;
     if (run) {
+      // the wave in its MFMA phase issues first: of the two waves on a SIMD,
+      // the other is copying, at the barrier or building A fragments
+      // (98.8 µs against 109.5 µs at 4096³ without it)
+      __builtin_amdgcn_s_setprio(1);
       const uint32_t sbase = sB0 + (uint32_t)(buf * SB) + rd_lane;
       // all B fragments of the stage are requested first (two K steps at
       // KS = 64), then the MFMAs run as they arrive
@@ -239,6 +243,7 @@ __device__ __forceinline__ void mfma4_tile(const uint32_t* __restrict__ W, int64
             }
         }
       }
+      __builtin_amdgcn_s_setprio(0);
     }
   }
 
