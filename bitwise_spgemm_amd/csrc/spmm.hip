@@ -53,6 +53,10 @@ bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_
                      const bws_plan* plan, hipStream_t s);
 size_t spmm_vals_workspace(const bws_bitmap* A);
 size_t spmm_split_workspace(int64_t K, int64_t N);
+size_t spmm_round_workspace(int64_t K, int64_t N);
+bws_status spmm_mfma_round(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
+                           int64_t ldc, float alpha, float beta, int32_t precision,
+                           const bws_plan* plan, void* workspace, hipStream_t s);
 bws_status spmm_mfma_split(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
                            int64_t ldc, float alpha, float beta, const bws_plan* plan,
                            void* workspace, hipStream_t s);
@@ -144,8 +148,14 @@ size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision
   (void)N;
   if (!A || A->rows < 0 || A->cols < 0 || N < 0) return 0;
   if (plan && plan->kernel == 1) return 0;
-  if (!A->values)
-    return precision == BWS_PREC_BF16X2 && N % 8 == 0 ? spmm_split_workspace(A->cols, N) : 0;
+  if (!A->values) {
+    if (N % 8 != 0) return 0;
+    if (precision == BWS_PREC_BF16X2) return spmm_split_workspace(A->cols, N);
+    // BF16 / FP16: room to round an FP32 B once (unused for 16-bit B)
+    if (precision == BWS_PREC_BF16 || precision == BWS_PREC_FP16)
+      return spmm_round_workspace(A->cols, N);
+    return 0;
+  }
   if (precision != BWS_PREC_BF16 && precision != BWS_PREC_FP16) return 0;
   return spmm_vals_workspace(A);
 }
@@ -177,6 +187,13 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
       workspace_bytes >= spmm_vals_workspace(A) &&
       (reinterpret_cast<uintptr_t>(workspace) & 255u) == 0)
     return spmm_mfma_vals(A, B, precision, ldb, N, C, ldc, alpha, beta, workspace, stream);
+  // BF16 / FP16 with FP32 B on a 0/1 A: with room for the rounded B, round it
+  // once and run the direct-to-LDS kernel; otherwise round in registers.
+  if (!A->values && A->cols > 0 && (precision == BWS_PREC_BF16 || precision == BWS_PREC_FP16) &&
+      b_dtype == BWS_DTYPE_F32 && (!plan || plan->kernel != 1) && N % 8 == 0 &&
+      ws_ok(workspace, workspace_bytes, spmm_round_workspace(A->cols, N)))
+    return spmm_mfma_round(A, static_cast<const float*>(B), ldb, N, C, ldc, alpha, beta, precision,
+                           plan, workspace, stream);
   // Split-BF16 on a 0/1 A: with room for B's two BF16 planes, split B once and
   // run the direct-to-LDS BF16 kernel twice; otherwise split in registers.
   if (!A->values && A->cols > 0 && precision == BWS_PREC_BF16X2 && (!plan || plan->kernel != 1) &&

@@ -792,9 +792,61 @@ k_split_bf16x2(const float* __restrict__ B, int64_t K, int64_t N, int64_t ldb,
 
 size_t split_plane_bytes(int64_t K, int64_t N) { return (size_t)ceil_div(K * N * 2, 256) * 256; }
 
+// FP32 B rounded once to the precision's 16-bit type (to nearest even, as the
+// register-staged kernels round it), four columns per thread (N % 8 == 0).
+template <int PREC, bool VEC>
+__global__ void __launch_bounds__(256)
+k_round_b16(const float* __restrict__ B, int64_t K, int64_t N, int64_t ldb,
+            uint2* __restrict__ out) {
+  const int64_t quads = N / 4, total = K * quads;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t k = i / quads, c = 4 * (i - k * quads);
+    const float* src = B + k * ldb + c;
+    float e[4];
+    if (VEC) {
+      const float4 v = *reinterpret_cast<const float4*>(src);
+      e[0] = v.x, e[1] = v.y, e[2] = v.z, e[3] = v.w;
+    } else {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) e[x] = src[x];
+    }
+    uint32_t h[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) h[x] = PREC == 2 ? f32_to_f16_rne(e[x]) : f32_to_bf16_rne(e[x]);
+    out[i] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+  }
+}
+
 }  // namespace
 
 size_t spmm_split_workspace(int64_t K, int64_t N) { return 2 * split_plane_bytes(K, N); }
+size_t spmm_round_workspace(int64_t K, int64_t N) { return split_plane_bytes(K, N); }
+
+// BF16 / FP16 with FP32 B on a 0/1 A: B is rounded once into the workspace
+// (row-major, ld N) and the direct-to-LDS 16-bit kernel multiplies it; the
+// rounding and the K order are those of the register-staged kernel, so the
+// result is the same bit for bit, without staging B through registers.
+bws_status spmm_mfma_round(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
+                           int64_t ldc, float alpha, float beta, int32_t precision,
+                           const bws_plan* plan, void* workspace, hipStream_t s) {
+  const int64_t K = A->cols;
+  uint16_t* Bh = static_cast<uint16_t*>(workspace);
+  const int64_t quads = K * (N / 4);
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(quads, 256), 16384);
+  const bool vec = ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0;
+  uint2* out = reinterpret_cast<uint2*>(Bh);
+  if (precision == BWS_PREC_FP16) {
+    if (vec) hipLaunchKernelGGL((k_round_b16<2, true>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+    else hipLaunchKernelGGL((k_round_b16<2, false>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+  } else {
+    if (vec) hipLaunchKernelGGL((k_round_b16<1, true>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+    else hipLaunchKernelGGL((k_round_b16<1, false>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+  }
+  BWS_TRY(launch_check());
+  return spmm_mfma(A, Bh, precision == BWS_PREC_FP16 ? BWS_DTYPE_F16 : BWS_DTYPE_BF16, N, N, C,
+                   ldc, alpha, beta, precision, plan, s);
+}
 
 // Split-BF16 through the 16-bit kernels: B's hi and lo planes are written to
 // the workspace (row-major, ld N) and multiplied by the BF16 MFMA kernel in

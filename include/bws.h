@@ -352,6 +352,10 @@ typedef struct {
  *     (C = alpha·A·hi + beta·C, then C += alpha·A·lo); without it B is split
  *     in registers inside one kernel.  The two forms differ by FP32 rounding
  *     only (one more rounding of the partial sum).
+ *   - BF16 / FP16 with a 0/1 A and N % 8 == 0: room for B rounded once to the
+ *     16-bit type, used when B is F32 (then multiplied by the direct-to-LDS
+ *     kernel; without it B is rounded in registers inside the kernel).  Both
+ *     forms give the same result bit for bit.
  * Semantics of the FP32 add path of backup/layout_trans.cu:145-171 and the
  * numeric kernel spgemm_compute_1dthread_tcore_v3 (source/src/transform.cuh:
  * 2062-2248) with the operand roles of a bitmap A and a dense B. */

@@ -145,15 +145,18 @@ def test_sddmm_and_groups_validation(bws):
 
 
 def test_workspace_sizes(bws):
-    """Workspace queries: BF16X2 on a 0/1 A asks for B's two BF16 planes (when
-    N % 8 == 0 and the plan allows MFMA), nothing otherwise; the SDDMM
-    workspace holds the per-word offsets and one byte per 64 x 64 tile."""
+    """Workspace queries: on a 0/1 A with N % 8 == 0 and a plan that allows
+    MFMA, BF16X2 asks for B's two BF16 planes and BF16 / FP16 for one rounded
+    copy of B (used when B is FP32), nothing otherwise; the SDDMM workspace
+    holds the per-word offsets and one byte per 64 x 64 tile."""
     L = bws.lib()
     P = bws.PRECISION
     a = _bm(300, 200, 7)
     assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16x2"], None) >= 2 * 200 * 64 * 2
     assert L.bws_spmm_workspace_size(C.byref(a), 60, P["bf16x2"], None) == 0
-    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16"], None) == 0
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16"], None) >= 200 * 64 * 2
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp16"], None) >= 200 * 64 * 2
+    assert L.bws_spmm_workspace_size(C.byref(a), 60, P["bf16"], None) == 0
     assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp32"], None) == 0
     plan = bws.make_plan(kernel=1)
     assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16x2"], C.byref(plan)) == 0

@@ -154,6 +154,33 @@ def test_spmm_bf16x2_two_forms(oracle, bws, dev, M, K, N, p, beta):
     assert d.max() <= 2e-6, d.max()
 
 
+@pytest.mark.parametrize("M,K,N,p", [(1000, 500, 264, 0.05), (257, 97, 136, 0.3), (300, 700, 520, 0.1)])
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_spmm_fp32_b_rounded_once(oracle, bws, dev, M, K, N, p, prec):
+    """FP32 B at BF16 / FP16: rounded once into the workspace (then the
+    direct-to-LDS kernel) or in registers (no workspace) — the same result bit
+    for bit, and the same as passing the rounded 16-bit B."""
+    import ctypes as C
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, p, seed=N + 5)
+    s = A.struct()
+    lib = bws.lib()
+    P = bws.PRECISION[prec]
+    wsz = lib.bws_spmm_workspace_size(C.byref(s), N, P, None)
+    assert wsz >= K * N * 2
+    outs = []
+    for ws in (None, bws._workspace(wsz, dev)):
+        out = torch.empty((M, N), device=dev)
+        st = lib.bws_spmm(C.byref(s), bws._ptr(Bd), 0, N, N, bws._ptr(out), N, C.c_float(1.0),
+                          C.c_float(0.0), P, None, bws._ptr(ws), 0 if ws is None else ws.numel(),
+                          bws._stream(dev))
+        assert st == 0
+        outs.append(out)
+    B16 = Bd.to(torch.bfloat16 if prec == "bf16" else torch.float16)
+    ref = bws.spmm(A, B16, prec)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[1], ref)
+
+
 @pytest.mark.parametrize("b_off", [4, 1])  # 16-byte aligned B rows (vector split) or not
 def test_spmm_bf16x2_strided_alpha_beta(oracle, bws, dev, rng, b_off):
     """Split BF16 through the workspace with strided B and C, alpha and beta."""
