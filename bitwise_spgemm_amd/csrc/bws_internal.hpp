@@ -88,10 +88,6 @@ struct BatchEntry {
 bws_status launch_mfma4_big(int32_t precision, const bws_bitmap* A, const void* B, int64_t ldb,
                             int64_t N, float* C, int64_t ldc, float alpha, float beta,
                             hipStream_t s);
-bws_status launch_mfma4_big_batch(int32_t precision, const BatchEntry* entries,
-                                  const int64_t* tile_off, int count, int64_t total_tiles,
-                                  hipStream_t s);
-
 // the 256 x 256-block 16-bit MFMA SDDMM (sddmm_mfma_big.hip)
 bws_status launch_sddmm_big(int32_t precision, const bws_bitmap* A, const int32_t* woff,
                             const void* dC, int64_t lddc, const void* B, int64_t ldb, int64_t N,

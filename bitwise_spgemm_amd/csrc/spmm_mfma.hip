@@ -640,9 +640,6 @@ bws_status launch_mfma_batch(const BatchEntry* entries, const int64_t* tile_off,
   const dim3 grid((unsigned)(8 * per_xcd));
   bool done = false;
   if constexpr (TB != 0 && PREC != 3) {
-    if (t.dma && t.bm == 256 && t.bn == 256)
-      return launch_mfma4_big_batch(PREC == 2 ? BWS_PREC_FP16 : BWS_PREC_BF16, entries, tile_off,
-                                    count, total_tiles, s);
     if (t.dma) {
       if (t.bn == 256)
         hipLaunchKernelGGL((k_spmm_mfma4_batch<PREC, 64, 128, 2, 64>), grid, dim3(256), 0, s,

@@ -5,7 +5,9 @@
 // traffic per MFMA of the 64 x 128 wave tile.  There is no empty-stage skip:
 // a branch around the MFMAs would make the compiler move the accumulators
 // between register files, so this tile is for masks whose stages all have
-// bits (it is chosen by plan or by the autotuner).
+// bits (it is chosen by plan or by the autotuner).  Grouped launches do not
+// use it: there the per-problem lookup pushes the wave past its register
+// budget, so they run a 256 x 256 request as 128 x 256.
 #include "mfma4_tile.hpp"
 
 namespace bws {
@@ -21,24 +23,6 @@ k_spmm_mfma4_big(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   if (tile >= row_blocks * col_blocks) return;
   mfma4_tile<PREC, 128, 128, 2, 64, false>(W, M, K, ldw, B, ldb, N, C, ldc, alpha, beta, tile,
                                            row_blocks);
-}
-
-template <int PREC>
-__global__ void __launch_bounds__(256, 1)
-k_spmm_mfma4_big_batch(const BatchEntry* __restrict__ entries,
-                       const int64_t* __restrict__ tile_off, int count, int64_t per_xcd) {
-  const int64_t g = xcd_tile(blockIdx.x, per_xcd);
-  if (g >= tile_off[count]) return;
-  int lo = 0, hi = count;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tile_off[mid] <= g) lo = mid;
-    else hi = mid;
-  }
-  const BatchEntry& e = entries[lo];
-  mfma4_tile<PREC, 128, 128, 2, 64, false>(e.W, e.M, e.K, e.ldw,
-                                           static_cast<const uint16_t*>(e.B), e.ldb, e.N, e.C,
-                                           e.ldc, 1.0f, 0.0f, g - tile_off[lo], e.row_blocks);
 }
 
 }  // namespace
@@ -58,22 +42,6 @@ bws_status launch_mfma4_big(int32_t precision, const bws_bitmap* A, const void* 
   else
     hipLaunchKernelGGL(k_spmm_mfma4_big<1>, grid, dim3(256), 0, s, A->words, A->rows, A->cols,
                        A->ldw, Bh, ldb, N, C, ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
-  return launch_check();
-}
-
-bws_status launch_mfma4_big_batch(int32_t precision, const BatchEntry* entries,
-                                  const int64_t* tile_off, int count, int64_t total_tiles,
-                                  hipStream_t s) {
-  const int64_t per_xcd = ceil_div(total_tiles, 8);
-  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
-  if (total_tiles == 0) return BWS_OK;
-  const dim3 grid((unsigned)(8 * per_xcd));
-  if (precision == BWS_PREC_FP16)
-    hipLaunchKernelGGL(k_spmm_mfma4_big_batch<2>, grid, dim3(256), 0, s, entries, tile_off, count,
-                       per_xcd);
-  else
-    hipLaunchKernelGGL(k_spmm_mfma4_big_batch<1>, grid, dim3(256), 0, s, entries, tile_off, count,
-                       per_xcd);
   return launch_check();
 }
 
