@@ -24,7 +24,7 @@ bws_status spmm_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, i
                            int32_t precision, int vec, hipStream_t s);
 bool valu_direct_ok(int32_t b_dtype, const void* B, int64_t ldb, int64_t N, const float* C,
                     int64_t ldc);
-ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N);
+ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N, bool vals);
 
 namespace {
 
@@ -121,8 +121,8 @@ Choice classify(const bws_problem& p, int32_t b_dtype, int32_t precision, const 
   }
   if (p.A.cols > 0 && valu_direct_ok(b_dtype, p.B, p.ldb, p.N, p.C, p.ldc)) {
     c.kind = 1;
-    c.cfg = plan_valu_cfg(eff.rows_per_wg ? &eff : nullptr, p.A.rows, p.N);
     c.vals = p.A.values != nullptr;
+    c.cfg = plan_valu_cfg(eff.rows_per_wg ? &eff : nullptr, p.A.rows, p.N, c.vals);
     return c;
   }
   c.kind = 3;
@@ -163,7 +163,7 @@ bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t
   if (!batch || count < 0 || (count > 0 && !problems)) return BWS_EINVAL;
   if (b_dtype < 0 || b_dtype > 2 || precision < 0 || precision > 3) return BWS_EINVAL;
   if (plan && (plan->kernel < 0 || plan->kernel > 2)) return BWS_EINVAL;
-  if (plan && plan->kernel != 2 && plan->rows_per_wg && !valu_cfg_valid(plan_valu_cfg(plan, 1, 1)))
+  if (plan && plan->kernel != 2 && plan->rows_per_wg && !valu_cfg_valid(plan_valu_cfg(plan, 1, 1, false)))
     return BWS_EINVAL;
   if (plan && plan->kernel == 2 && !mfma_tile_valid(plan->rows_per_wg, plan->cols_per_wg))
     return BWS_EINVAL;

@@ -22,12 +22,16 @@ bool valu_cfg_valid(const ValuCfg& c) {
 // always 16 waves.  Measured on 4096 x 4096 x {1024, 2048, 4096}, 2048² x 4096,
 // 8192 x 4096 x 1024 and 16384 x 4096 x 512 with 5% clustered and 10% iid
 // masks: 16 waves beat 8 waves on the same rows by 25-45% (four waves per
-// SIMD hide the LDS and bit-scan latency of the per-row chains).
-ValuCfg default_valu_cfg(int64_t M, int64_t N) {
+// SIMD hide the LDS and bit-scan latency of the per-row chains).  Value masks
+// stop at 8 rows per wave: with 16 rows the per-row values of two stages
+// exceed the 128-VGPR budget of a 1024-lane workgroup and spill, and 16 x 8
+// measured 1-3% faster than that on 4096³ (2%, 10%), 8192 x 4096 x 2048 and
+// 16384 x 4096 x 1024 value masks.
+ValuCfg default_valu_cfg(int64_t M, int64_t N, bool vals) {
   ValuCfg c;
   c.waves = 16;
   const int64_t tiles256 = ceil_div(M, 256) * ceil_div(N, 256);
-  if (tiles256 >= 192) return c;  // 16 waves x 16 rows
+  if (tiles256 >= 192 && !vals) return c;  // 16 waves x 16 rows
   const int64_t tiles128 = ceil_div(M, 128) * ceil_div(N, 256);
   if (tiles128 >= 192) {
     c.rows = 128;
@@ -39,8 +43,8 @@ ValuCfg default_valu_cfg(int64_t M, int64_t N) {
 }
 
 // The plan's vector-add tile, or the default where the plan leaves it 0.
-ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N) {
-  ValuCfg c = default_valu_cfg(M, N);
+ValuCfg plan_valu_cfg(const bws_plan* plan, int64_t M, int64_t N, bool vals) {
+  ValuCfg c = default_valu_cfg(M, N, vals);
   if (plan && plan->rows_per_wg) {
     c.rows = plan->rows_per_wg;
     c.waves = plan->waves_per_wg ? plan->waves_per_wg : 16;
@@ -134,7 +138,8 @@ bws_status spmm_dispatch(const bws_bitmap* A, const void* B, int32_t b_dtype, in
     if (A->values) return BWS_EUNSUPPORTED;
     return spmm_mfma(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, precision, plan, s);
   }
-  return spmm_valu(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, plan_valu_cfg(plan, M, N), s);
+  const ValuCfg cfg = plan_valu_cfg(plan, M, N, A->values != nullptr);
+  return spmm_valu(A, B, b_dtype, ldb, N, C, ldc, alpha, beta, cfg, s);
 }
 
 }  // namespace bws
@@ -169,7 +174,7 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   if (!combo_ok(precision, b_dtype)) return BWS_EUNSUPPORTED;
   if (plan && (plan->kernel < 0 || plan->kernel > 2 || plan->rows_per_wg < 0 ||
                (plan->kernel != 2 && plan->rows_per_wg != 0 &&
-                !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N))) ||
+                !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N, A->values != nullptr))) ||
                (plan->kernel == 2 && !mfma_tile_valid(plan->rows_per_wg, plan->cols_per_wg))))
     return BWS_EINVAL;
   if (A->rows == 0 || N == 0) return BWS_OK;
