@@ -33,35 +33,84 @@ constexpr int PER_STAGE = NA_W + NB_W;
 constexpr int NBUF = 2;
 
 // Dense 16-bit A: one thread per word of the padded row (lda / 32 words).
+//  - A wave's 64 words are consecutive (row, word) positions and values are
+//    stored row by row, so the wave's values are one contiguous run, from its
+//    first word with a set bit to the end of its last (at most 2048).  The
+//    wave copies that run into LDS with coalesced loads, rounding each value
+//    once, and every lane reads its word's values from there by rank.
+//  - Since lda = 32 x (words per padded row), thread t's 64 output bytes sit
+//    at byte 64 t of Ad: the workgroup's output is one contiguous 16 KiB run,
+//    staged in LDS (over the values) and stored 4 KiB per instruction.
 template <int PREC>
 __global__ void __launch_bounds__(256)
 k_expand_values(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
                 const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ woff,
                 const float* __restrict__ values, uint16_t* __restrict__ Ad, int64_t lda) {
+  __shared__ uint4 stage[256 * 4];
   const int64_t wpr = words_for(K);
   const int64_t wpad = lda / 32;
   const int64_t n = M * wpad;
   const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  uint16_t* sv = reinterpret_cast<uint16_t*>(stage) + 2048 * (tid >> 6);  // this wave's values
+  for (int64_t t0 = (int64_t)blockIdx.x * 256; t0 < n; t0 += stride) {
+    const int64_t t = t0 + tid;
     const int64_t r = t / wpad, w = t - r * wpad;
+    uint32_t m = 0u;
+    int64_t e = 0;
+    if (t < n && w < wpr) {
+      m = W[r * ldw + w] & valid_bits(w, K);
+      e = row_ptr[r] + woff[r * wpr + w];
+    }
+    const uint64_t act = __ballot(m != 0u);
+    int64_t e0 = 0;
+    __syncthreads();  // the previous run's stores have read the stage
+    if (act) {
+      const int first = __ffsll((unsigned long long)act) - 1;
+      const int last = 63 - __clzll((long long)act);
+      e0 = ((int64_t)__builtin_amdgcn_readlane((int)(e >> 32), first) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)e, first);
+      const int64_t e_end = e + __popc(m);
+      const int64_t e1 = ((int64_t)__builtin_amdgcn_readlane((int)(e_end >> 32), last) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int)e_end, last);
+      const int cnt = e1 - e0 < 2048 ? (int)(e1 - e0) : 2048;  // 2048 unless row_ptr is not A's
+      for (int k = lane; k < cnt; k += 64)
+        sv[k] = (uint16_t)(PREC == 2 ? f32_to_f16_rne(values[e0 + k])
+                                     : f32_to_bf16_rne(values[e0 + k]));
+    }
+    __syncthreads();
+    // bit i's value sits at (set bits before i) past the word's first; every
+    // position reads (a clear bit at an index clamped into the word's own
+    // values, 0 for an empty word) and the clear ones are zeroed afterwards
     uint32_t d[16];
+    {
+      const int base = m ? (int)(e - e0) : 0;
+      const int top = m ? __popc(m) - 1 : 0;
+      auto at = [&](int i) -> uint32_t {
+        const int before = i == 0 ? 0 : __popc(m >> (32 - i));
+        return sv[base + (before < top ? before : top)];
+      };
 #pragma unroll
-    for (int i = 0; i < 16; ++i) d[i] = 0u;
-    if (w < wpr) {
-      const uint32_t m = W[r * ldw + w] & valid_bits(w, K);
-      int64_t e = row_ptr[r] + woff[r * wpr + w];
+      for (int i = 0; i < 16; ++i) d[i] = at(2 * i) | (at(2 * i + 1) << 16);
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        if (m & (0x80000000u >> i)) {
-          const float v = values[e++];
-          const uint32_t h = PREC == 2 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
-          d[i >> 1] |= h << (16 * (i & 1));
-        }
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t keep = ((m & (0x80000000u >> (2 * i))) ? 0x0000FFFFu : 0u) |
+                              ((m & (0x40000000u >> (2 * i))) ? 0xFFFF0000u : 0u);
+        d[i] &= keep;
       }
     }
-    uint4* dst = reinterpret_cast<uint4*>(Ad + r * lda + 32 * w);
+    __syncthreads();  // every wave has read its values (the stage overlays them)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) dst[i] = make_uint4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+    for (int i = 0; i < 4; ++i)
+      stage[4 * tid + i] = make_uint4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+    __syncthreads();
+    const int64_t chunks = 4 * (n - t0 < 256 ? n - t0 : 256);
+    uint4* dst = reinterpret_cast<uint4*>(Ad) + 4 * t0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = 256 * i + tid;
+      if (c < chunks) dst[c] = stage[c];
+    }
   }
 }
 

@@ -256,6 +256,31 @@ def test_spmm_mfma_value_mask(oracle, bws, dev, M, K, N, p, prec, dt):
     assert rel_err(oracle, out.cpu().numpy(), W, K, Bx, rp, vals)["max_rel_s"] <= 1e-5
 
 
+@pytest.mark.parametrize("prec,dt", [("bf16", torch.bfloat16), ("fp16", torch.float16)])
+def test_spmm_mfma_value_mask_banded(oracle, bws, dev, prec, dt):
+    """Value-carrying A whose set bits come in bands: 200 empty rows (whole
+    waves of words without a value), 64 full rows (2048 values per wave of
+    words), rows holding only their last column, and the matrix's last bit
+    set, so the final value sits at the end of the value array."""
+    M, K, N = 520, 1000, 72
+    D = np.zeros((M, K), dtype=np.float32)
+    D[200:264, :] = 1.0
+    D[264:400, K - 1] = 1.0
+    D[400:, 3::37] = 1.0
+    D[M - 1, K - 1] = 1.0
+    W = oracle.pack_bitmap(D)
+    rp = oracle.row_ptr(W, K)
+    nnz = int(rp[-1])
+    vals = bws.gen_f32(5, nnz, "normal", 0.0, 0.02)
+    vals[vals == 0] = 1e-3
+    A = bws.Bitmap(to_dev(W, dev), M, K, to_dev(rp, dev), to_dev(vals, dev))
+    B = bws.gen_f32(6, (K, N), "normal", 0.0, 1.0)
+    rnd = oracle.round_bf16 if prec == "bf16" else oracle.round_f16
+    C = bws.spmm(A, to_dev(B, dev).to(dt), prec).cpu().numpy()
+    assert rel_err(oracle, C, W, K, rnd(B), rp, rnd(vals))["max_rel_s"] <= 1e-5
+    assert not C[:200].any()
+
+
 def test_spmm_mfma_tile_rejects_bad_tile(bws, dev):
     A = bws.pack_bitmap(torch.ones((64, 64), device=dev))
     B = torch.ones((64, 64), device=dev, dtype=torch.bfloat16)
