@@ -113,3 +113,13 @@ def test_transpose_perm_and_gather(oracle, bws, dev, rng, rows, cols, p):
     vT = bws.gather(to_dev(vals, dev), perm)
     assert np.array_equal(vT.cpu().numpy(), vt)
 
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 4099, 1 << 20])
+def test_gather_lengths(bws, dev, rng, n):
+    """dst[i] = src[idx[i]] for short, odd and long lengths, with repeated
+    indices."""
+    src = rng.standard_normal(n + 7).astype(np.float32)
+    idx = rng.integers(0, n + 7, size=n).astype(np.int64)
+    got = bws.gather(to_dev(src, dev), to_dev(idx, dev)).cpu().numpy()
+    assert np.array_equal(got, src[idx])
+
