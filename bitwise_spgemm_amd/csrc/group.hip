@@ -39,16 +39,16 @@ k_group_rows(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t
     uint32_t gm_lane = 0;  // lane g < G holds the mask of group g
     int32_t cnt = 0;       // spilled rows so far (wave-uniform)
     int32_t snnz_lane = 0;
-    for (int j0 = 0; j0 < nrows; j0 += 64) {
+    // one chunk of 64 rows, in row order (the first-fit order)
+    auto chunk = [&](int j0, uint32_t m) {
       const int j = j0 + lane;
       const bool valid = j < nrows;
-      const uint32_t m = valid ? (W[(row0 + j) * ldw + w] & valid_bits(w, cols)) : 0u;
       uint64_t pending = __ballot(m != 0u);
       int32_t gid = 0, slot = -1;
       while (pending) {
         int fit = G;
         for (int g = 0; g < G; ++g) {
-          const uint32_t gmg = __shfl(gm_lane, g, kWave);
+          const uint32_t gmg = (uint32_t)__builtin_amdgcn_readlane((int)gm_lane, g);
           if (fit == G && (gmg & m) == 0u) fit = g;
         }
         const bool pend = (pending >> lane) & 1ull;
@@ -63,8 +63,10 @@ k_group_rows(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t
         cnt += __popcll(spillers);
         pending &= ~spillers;
         if (!joiners) break;
-        const int gj = __shfl(fit, jl, kWave);
-        const uint32_t mj = __shfl(m, jl, kWave);
+        // jl and g are wave-uniform: v_readlane, not an LDS permute, on the
+        // placement's serial chain
+        const int gj = __builtin_amdgcn_readlane(fit, jl);
+        const uint32_t mj = (uint32_t)__builtin_amdgcn_readlane((int)m, jl);
         if (lane == jl) gid = gj;
         if (lane == gj) gm_lane |= mj;
         int64_t vbase = 0;
@@ -86,13 +88,33 @@ k_group_rows(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t
         if (spill_rev) spill_rev[t * T + j] = slot;
         if (spill_table && slot >= 0) spill_table[t * T + slot] = j;
       }
+    };
+    // the words of PF chunks are loaded before the first of them is placed:
+    // the placement is sequential, the loads need not be (one load latency
+    // per PF chunks instead of one per chunk)
+    constexpr int PF = 8;
+    const uint32_t vb = valid_bits(w, cols);
+    for (int b0 = 0; b0 < nrows; b0 += 64 * PF) {
+      // unconditional loads (rows past the tile read its last row and are
+      // zeroed afterwards): a load under a branch waits where the branch joins
+      uint32_t mm[PF];
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int j = b0 + 64 * u + lane;
+        mm[u] = W[(row0 + (j < nrows ? j : nrows - 1)) * ldw + w];
+      }
+#pragma unroll
+      for (int u = 0; u < PF; ++u) mm[u] = b0 + 64 * u + lane < nrows ? mm[u] & vb : 0u;
+#pragma unroll
+      for (int u = 0; u < PF; ++u)
+        if (b0 + 64 * u < nrows) chunk(b0 + 64 * u, mm[u]);
     }
     if (spill_rev)
       for (int j = nrows + lane; j < T; j += 64) spill_rev[t * T + j] = -1;
     if (spill_table)
       for (int s = cnt + lane; s < T; s += 64) spill_table[t * T + s] = 0;
     for (int g = 0; g < G; ++g) {
-      const uint32_t gmg = __shfl(gm_lane, g, kWave);
+      const uint32_t gmg = (uint32_t)__builtin_amdgcn_readlane((int)gm_lane, g);
       if (lane < 32 && !(gmg & wbit(lane))) {
         const int64_t idx = (t * G + g) * 32 + lane;
         group_vals[idx] = 0.0f;
@@ -143,7 +165,7 @@ k_group_spilled_csr(const uint32_t* __restrict__ W, int64_t cols, int64_t wpr, i
         val[e] = HAS_VALUES ? values[vb++] : 1.0f;
         ++e;
       }
-      carry += __shfl(incl, 63, kWave);
+      carry += __builtin_amdgcn_readlane(incl, 63);
     }
   }
 }
