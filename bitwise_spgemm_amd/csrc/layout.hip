@@ -73,9 +73,20 @@ k_pack_bitmap_f32_vec(const float* __restrict__ D, int64_t rows, int64_t cols, i
     const float* row = D + r * ld;
     float4 v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t c = c0 + 256 * u + 4 * lane;
-      v[u] = c < cols ? *reinterpret_cast<const float4*>(row + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int u = 0; u < 4; ++u) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (c0 < cols) {  // wave-uniform; then cols >= 4
+      // unconditional loads (columns past the row read its last four and are
+      // zeroed afterwards): a load under a per-lane branch waits where the
+      // branch joins, which would leave one load in flight instead of four
+      float4 x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t c = c0 + 256 * u + 4 * lane;
+        x[u] = *reinterpret_cast<const float4*>(row + (c < cols ? c : cols - 4));
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (c0 + 256 * u + 4 * lane < cols) v[u] = x[u];
     }
     uint64_t b[4][4];
 #pragma unroll
