@@ -26,11 +26,19 @@ k_pack_bitmap_f32(const float* __restrict__ D, int64_t rows, int64_t cols, int64
     const int64_t r = gw / chunks_per_row;
     const int64_t c0 = (gw - r * chunks_per_row) * 256;
     const float* row = D + r * ld;
-    float v[4];
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (c0 < cols) {  // wave-uniform
+      // unconditional loads (columns past the row read its last one and are
+      // zeroed afterwards), so that all four are in flight together
+      float x[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t c = c0 + 64 * u + lane;
-      v[u] = c < cols ? row[c] : 0.0f;
+      for (int u = 0; u < 4; ++u) {
+        const int64_t c = c0 + 64 * u + lane;
+        x[u] = row[c < cols ? c : cols - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (c0 + 64 * u + lane < cols) v[u] = x[u];
     }
     uint64_t b[4];
 #pragma unroll
