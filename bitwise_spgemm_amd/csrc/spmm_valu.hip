@@ -507,10 +507,12 @@ k_spmm_groups(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
         const float a = g >= 0 ? gv[(t * G + g) * 32 + i] : (values ? values[e] : 1.0f);
         ++e;
         const float* b = B + (32 * w + i) * ldb;
+        // lanes past column N read the last column and never store: the four
+        // loads carry no branch, so they are in flight together
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int64_t n = cb * 256 + q * 64 + lane;
-          if (n < N) acc[q] = fmaf(a, b[n], acc[q]);
+          acc[q] = fmaf(a, b[n < N ? n : N - 1], acc[q]);
         }
       }
     }
