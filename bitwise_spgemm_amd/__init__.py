@@ -130,8 +130,10 @@ _SIGS = {
     "bws_group_spilled_csr": ([_BP, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P], _I32),
     "bws_group_scatter": ([_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _I64, _P], _I32),
     "bws_group_gather": ([_P, _I64, _P, _I64, _I64, _I64, _P, _I32, _I32, _P, _P], _I32),
-    "bws_cluster_workspace_size": ([_I64, _I32], _SZ),
+    "bws_cluster_workspace_size": ([_I64, _I64, _I32], _SZ),
     "bws_cluster_rows": ([_BP, _I32, _P, _P, _P, _SZ, _P], _I32),
+    "bws_cluster_stream_size": ([_I64, _I64], _SZ),
+    "bws_cluster_stream": ([_BP, _P, _P, _P, _SZ, _P], _I32),
     "bws_spmm_clustered": ([_BP, _P, _P, _P, _I64, _I64, _P, _I64, _F, _F, _P], _I32),
     "bws_spmm_workspace_size": ([_BP, _I64, _I32, C.POINTER(_Plan)], _SZ),
     "bws_spmm": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P, _SZ,
@@ -155,8 +157,11 @@ _SIGS = {
     "bws_autotune_save": ([C.c_char_p], _I32),
     "bws_autotune_load": ([C.c_char_p], _I32),
     "bws_symbolic_or": ([_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _P], _I32),
+    "bws_spgemm_workspace_size": ([_I64, _I64, _I64], _SZ),
+    "bws_spgemm": ([_BP, _BP, _BP, _F, _P, _P, _SZ, _P], _I32),
+    "bws_spmm_bitplanes_workspace_size": ([_I64, _I64, _I32], _SZ),
     "bws_spmm_bitplanes_i8": ([_P, _I64, _I64, _I64, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32,
-                               C.POINTER(_Plan), _P], _I32),
+                               C.POINTER(_Plan), _P, _SZ, _P], _I32),
     "bws_gen_bitmap_iid": ([_U64, _I64, _I64, _D, _P, _I64], _I32),
     "bws_gen_bitmap_clustered": ([_U64, _I64, _I64, _I32, _D, _D, _P, _I64], _I32),
     "bws_gen_f32": ([_U64, _I64, _I32, _D, _D, _P], _I32),
@@ -601,27 +606,34 @@ SLOT_BASE1 = 1 << 29
 class RowClusters:
     """Row-clustering plan of a 0/1 bitmap (bws_cluster_rows): slot_row int32
     [rows] (row | SLOT_DELTA for rows summed as a difference from a chunk
-    base, | SLOT_BASE1 when that is the chunk's second base) and chunk_mask
+    base, | SLOT_BASE1 when that is the chunk's second base), chunk_mask
     int32 [ceil(rows/16), 2, ldw] (the chunks' two bases, uint32 words held as
-    int32)."""
+    int32) and the operand stream of the clustered product (bws_cluster_stream,
+    uint8 bytes)."""
 
-    def __init__(self, slot_row: torch.Tensor, chunk_mask: torch.Tensor, rows: int, cols: int,
-                 max_clusters: int):
+    def __init__(self, slot_row: torch.Tensor, chunk_mask: torch.Tensor, stream: torch.Tensor,
+                 rows: int, cols: int, max_clusters: int):
         self.slot_row = slot_row
         self.chunk_mask = chunk_mask
+        self.stream = stream
         self.rows, self.cols, self.max_clusters = rows, cols, max_clusters
 
 
 def cluster_rows(A: Bitmap, max_clusters: int = 0) -> RowClusters:
-    """Row clustering for spmm_clustered (0/1 masks; max_clusters 0 = rows / 64)."""
+    """Row clustering for spmm_clustered (0/1 masks; max_clusters 0 = rows / 64),
+    with the product's operand stream."""
     dev = A.words.device
     slot_row = torch.empty(max(A.rows, 1), dtype=torch.int32, device=dev)[:A.rows]
     chunk_mask = torch.empty(((A.rows + 15) // 16, 2, A.ldw), dtype=torch.int32, device=dev)
-    ws = _workspace(lib().bws_cluster_workspace_size(A.rows, max_clusters), dev)
+    ws = _workspace(lib().bws_cluster_workspace_size(A.rows, A.cols, max_clusters), dev)
     s = A.struct()
     _check(lib().bws_cluster_rows(C.byref(s), max_clusters, _ptr(slot_row), _ptr(chunk_mask),
                                   _ptr(ws), ws.numel(), _stream(dev)), "bws_cluster_rows")
-    return RowClusters(slot_row, chunk_mask, A.rows, A.cols, max_clusters)
+    nbytes = lib().bws_cluster_stream_size(A.rows, A.cols)
+    stream = _workspace(nbytes, dev)
+    _check(lib().bws_cluster_stream(C.byref(s), _ptr(slot_row), _ptr(chunk_mask), _ptr(stream),
+                                    stream.numel(), _stream(dev)), "bws_cluster_stream")
+    return RowClusters(slot_row, chunk_mask, stream, A.rows, A.cols, max_clusters)
 
 
 def spmm_clustered(A: Bitmap, clusters: RowClusters, B: torch.Tensor, alpha: float = 1.0,
@@ -637,9 +649,9 @@ def spmm_clustered(A: Bitmap, clusters: RowClusters, B: torch.Tensor, alpha: flo
             raise ValueError("beta != 0 needs `out`")
         out = torch.empty((A.rows, N), dtype=torch.float32, device=dev)
     s = A.struct()
-    _check(lib().bws_spmm_clustered(C.byref(s), _ptr(clusters.slot_row),
-                                    _ptr(clusters.chunk_mask), _ptr(B), B.stride(0), N,
-                                    _ptr(out), out.stride(0), alpha, beta, _stream(dev)),
+    _check(lib().bws_spmm_clustered(C.byref(s), _ptr(clusters.slot_row), _ptr(clusters.stream),
+                                    _ptr(B), B.stride(0), N, _ptr(out), out.stride(0), alpha,
+                                    beta, _stream(dev)),
            "bws_spmm_clustered")
     return out
 
@@ -804,18 +816,45 @@ def symbolic_or(A: Bitmap, Bm: Bitmap) -> Bitmap:
     return Bitmap(WC, A.rows, Bm.cols)
 
 
+def spgemm(A: Bitmap, Bm: Bitmap, Cm: Optional[Bitmap] = None, alpha: float = 1.0) -> Bitmap:
+    """Numeric sparse x sparse product: C = alpha * A·Bm on the pattern Cm (by
+    default the symbolic product of A's and Bm's patterns), returned as a
+    value-carrying bitmap.  bws.group_rows on it gives the grouped form."""
+    assert A.cols == Bm.rows
+    dev = A.words.device
+    if Cm is None:
+        Cm = symbolic_or(A, Bm)
+    if Cm.row_ptr is None:
+        Cm = Bitmap(Cm.words, Cm.rows, Cm.cols, row_ptr(Cm))
+    nnz = int(Cm.row_ptr[-1].item()) if Cm.rows else 0
+    store = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev)  # never a null pointer
+    ws = _workspace(lib().bws_spgemm_workspace_size(A.rows, A.cols, Bm.cols), dev)
+    a, b, c = A.struct(), Bm.struct(), Cm.struct()
+    _check(lib().bws_spgemm(C.byref(a), C.byref(b), C.byref(c), alpha, _ptr(store), _ptr(ws),
+                            ws.numel(), _stream(dev)), "bws_spgemm")
+    return Bitmap(Cm.words, Cm.rows, Cm.cols, Cm.row_ptr, store[:nnz])
+
+
 def spmm_bitplanes_i8(P: torch.Tensor, cols: int, B: torch.Tensor, precision: str = "fp32",
-                      alpha: float = 1.0, beta: float = 0.0, out=None, plan=None) -> torch.Tensor:
-    """C = Σ_b w_b P_b·B for int8 bit-planes P [8, M, ldw] (weights 1..64, -128)."""
+                      alpha: float = 1.0, beta: float = 0.0, out=None, plan=None,
+                      one_pass: bool = True) -> torch.Tensor:
+    """C = Σ_b w_b P_b·B for int8 bit-planes P [8, M, ldw] (weights 1..64, -128).
+    one_pass: give the library its workspace (B read once); False runs the
+    eight weighted 0/1 products."""
     _, M, ldw = P.shape
     N = B.shape[1]
     if out is None:
         out = torch.empty((M, N), dtype=torch.float32, device=B.device)
+    ws = None
+    if one_pass:
+        ws = _workspace(lib().bws_spmm_bitplanes_workspace_size(M, cols, PRECISION[precision]),
+                        B.device)
     _check(lib().bws_spmm_bitplanes_i8(_ptr(P), M, cols, ldw, _ptr(B), DTYPE[B.dtype], B.stride(0),
                                        N, _ptr(out), out.stride(0), alpha, beta,
                                        PRECISION[precision],
-                                       None if plan is None else C.byref(plan),
-                                       _stream(B.device)), "bws_spmm_bitplanes_i8")
+                                       None if plan is None else C.byref(plan), _ptr(ws),
+                                       0 if ws is None else ws.numel(), _stream(B.device)),
+           "bws_spmm_bitplanes_i8")
     return out
 
 

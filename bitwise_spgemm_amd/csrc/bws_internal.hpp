@@ -94,6 +94,20 @@ bws_status launch_sddmm_big(int32_t precision, const bws_bitmap* A, const int32_
                             float* dA, hipStream_t s);
 int64_t sddmm_big_tiles(int64_t rows, int64_t cols);
 
+// The streamed FP32 0/1 product (spmm_stream.hip): the mask stream of a
+// plain bitmap or of a row-clustering plan, and the product over it.
+size_t mask_stream_bytes(int64_t M, int64_t K);
+size_t cluster_stream_bytes(int64_t M, int64_t K);
+bws_status launch_mask_stream(const bws_bitmap* A, uint32_t* S, hipStream_t s);
+bws_status launch_cluster_stream(const bws_bitmap* A, const int32_t* slot_row,
+                                 const uint32_t* chunk_mask, uint32_t* S, hipStream_t s);
+bws_status launch_spmm_stream(const uint32_t* S, const int32_t* slot_row, int64_t M, int64_t K,
+                              const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
+                              float alpha, float beta, hipStream_t s);
+// Rows and columns of C per workgroup of the streamed product.
+constexpr int64_t kStreamTileRows = 256;
+constexpr int64_t kStreamTileCols = 256;
+
 // Device scratch carving: a linear allocator over the caller's workspace.
 struct Carve {
   char* base;

@@ -112,6 +112,15 @@ bool combo_ok(int32_t precision, int32_t b_dtype) {
 
 size_t esize(int32_t d) { return d == BWS_DTYPE_F32 ? 4 : 2; }
 
+// The streamed FP32 kernel: 16-byte rows of B and C, and enough 256 x 256
+// tiles to occupy the chip (one workgroup per CU).
+bool stream_ok(const bws_bitmap* A, const void* B, int64_t ldb, int64_t N, const float* C,
+               int64_t ldc) {
+  if (N < 4 || N % 4 != 0 || ldb % 4 != 0 || ldc % 4 != 0) return false;
+  if ((reinterpret_cast<uintptr_t>(B) & 15u) || (reinterpret_cast<uintptr_t>(C) & 15u)) return false;
+  return ceil_div(A->rows, kStreamTileRows) * ceil_div(N, kStreamTileCols) >= 192;
+}
+
 }  // namespace
 
 bws_status spmm_dispatch(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb,
@@ -152,6 +161,8 @@ size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision
                                const bws_plan* plan) {
   (void)N;
   if (!A || A->rows < 0 || A->cols < 0 || N < 0) return 0;
+  if (precision == BWS_PREC_FP32 && !A->values && N % 4 == 0 && (!plan || plan->rows_per_wg == 0))
+    return mask_stream_bytes(A->rows, A->cols);
   if (plan && plan->kernel == 1) return 0;
   if (!A->values) {
     if (N % 8 != 0) return 0;
@@ -183,6 +194,17 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   if ((reinterpret_cast<uintptr_t>(C) & 3u) ||
       (reinterpret_cast<uintptr_t>(B) & (esize(b_dtype) - 1)))
     return BWS_EINVAL;
+  // FP32 on a 0/1 A with the mask stream in the workspace: the streamed
+  // kernel (row masks walked on the scalar unit), for problems that fill the
+  // chip with 256 x 256 tiles
+  if (precision == BWS_PREC_FP32 && !A->values && A->cols > 0 && b_dtype == BWS_DTYPE_F32 &&
+      (!plan || plan->rows_per_wg == 0) && stream_ok(A, B, ldb, N, C, ldc) &&
+      ws_ok(workspace, workspace_bytes, mask_stream_bytes(A->rows, A->cols))) {
+    uint32_t* S = static_cast<uint32_t*>(workspace);
+    BWS_TRY(launch_mask_stream(A, S, stream));
+    return launch_spmm_stream(S, nullptr, A->rows, A->cols, static_cast<const float*>(B), ldb, N,
+                              C, ldc, alpha, beta, stream);
+  }
   // A value-carrying A on MFMA needs the dense 16-bit copy of A in the
   // workspace; without it (or for B layouts the direct-to-LDS kernel cannot
   // take) the FP32 vector path runs, which is at least as accurate.

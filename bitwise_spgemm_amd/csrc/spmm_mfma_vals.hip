@@ -1,5 +1,6 @@
-// spmm_mfma_vals.hip — 16-bit MFMA SpMM for a value-carrying bitmap A
-// (gfx950, v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulate).
+// spmm_mfma_vals.hip — 16-bit MFMA SpMM for a value-carrying bitmap A and for
+// an int8 A held as bit-planes (gfx950, v_mfma_f32_16x16x32_{bf16,f16}, FP32
+// accumulate).
 //
 // A 0/1 operand is built from the bitmap bits inside the MFMA kernels; a
 // value-carrying A needs each set bit's own value in the fragment, a gather
@@ -111,6 +112,38 @@ k_expand_values(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ld
       const int c = 256 * i + tid;
       if (c < chunks) dst[c] = stage[c];
     }
+  }
+}
+
+// Dense 16-bit A from int8 bit-planes P[8][M][ldw]: a thread per (row, word):
+// value(c) = Σ_b w_b·bit_b(c) with w = [1, 2, 4, ..., 64, -128] (exact in
+// BF16 and FP16: integers of magnitude <= 128), written as 32 16-bit values.
+template <int PREC>
+__global__ void __launch_bounds__(256)
+k_expand_planes(const uint32_t* __restrict__ P, int64_t M, int64_t K, int64_t ldw,
+                uint16_t* __restrict__ Ad, int64_t lda) {
+  const int64_t wpr = words_for(K);
+  const int64_t wpad = lda / 32;
+  const int64_t n = M * wpad;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int64_t r = t / wpad, w = t - r * wpad;
+    uint32_t pl[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      pl[b] = w < wpr ? P[((int64_t)b * M + r) * ldw + w] & valid_bits(w, K) : 0u;
+    uint32_t d[16];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      int v = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) v += ((pl[b] >> (31 - i)) & 1u) ? (b == 7 ? -128 : (1 << b)) : 0;
+      const uint32_t h = PREC == 2 ? f32_to_f16_rne((float)v) : f32_to_bf16_rne((float)v);
+      if (i & 1) d[i >> 1] |= h << 16;
+      else d[i >> 1] = h;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(Ad + r * lda + 32 * w);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dst[i] = make_uint4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
   }
 }
 
@@ -259,6 +292,50 @@ size_t spmm_vals_workspace(const bws_bitmap* A) {
   return cv.used + 256;
 }
 
+size_t spmm_planes_workspace(int64_t M, int64_t K) {
+  return align_up((size_t)(M * padded_lda(K)) * 2, 256) + 256;
+}
+
+// The 16-bit MFMA product of a dense 16-bit A (rows padded to lda) already in
+// the workspace.
+static bws_status launch_vals_product(const uint16_t* Ad, int64_t M, int64_t K, int64_t lda,
+                                      const void* B, int32_t precision, int64_t ldb, int64_t N,
+                                      float* C, int64_t ldc, float alpha, float beta,
+                                      hipStream_t s) {
+  const int64_t row_blocks = ceil_div(M, VBM), col_blocks = ceil_div(N, VBN);
+  const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  const dim3 grid((unsigned)(8 * per_xcd));
+  const uint16_t* Bh = static_cast<const uint16_t*>(B);
+  if (precision == BWS_PREC_FP16)
+    hipLaunchKernelGGL(k_spmm_mfma_vals<2>, grid, dim3(256), 0, s, Ad, M, K, lda, Bh, ldb, N, C,
+                       ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
+  else
+    hipLaunchKernelGGL(k_spmm_mfma_vals<1>, grid, dim3(256), 0, s, Ad, M, K, lda, Bh, ldb, N, C,
+                       ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
+  return launch_check();
+}
+
+// int8 bit-planes on MFMA in one pass over B: the planes folded into a dense
+// 16-bit A (exact), then one product.  The caller checked precision, B
+// (dma_ok) and the workspace size.
+bws_status spmm_mfma_planes(const uint32_t* P, int64_t M, int64_t K, int64_t ldw, const void* B,
+                            int32_t precision, int64_t ldb, int64_t N, float* C, int64_t ldc,
+                            float alpha, float beta, void* workspace, hipStream_t s) {
+  const int64_t lda = padded_lda(K);
+  uint16_t* Ad = static_cast<uint16_t*>(workspace);
+  int64_t g = ceil_div(M * (lda / 32), 256);
+  if (g > 65536) g = 65536;
+  if (precision == BWS_PREC_FP16)
+    hipLaunchKernelGGL(k_expand_planes<2>, dim3((unsigned)g), dim3(256), 0, s, P, M, K, ldw, Ad,
+                       lda);
+  else
+    hipLaunchKernelGGL(k_expand_planes<1>, dim3((unsigned)g), dim3(256), 0, s, P, M, K, ldw, Ad,
+                       lda);
+  BWS_TRY(launch_check());
+  return launch_vals_product(Ad, M, K, lda, B, precision, ldb, N, C, ldc, alpha, beta, s);
+}
+
 // Value-carrying A on MFMA; the caller checked precision (BF16 / FP16), a
 // 16-bit B that dma_ok() accepts, and the workspace size.
 bws_status spmm_mfma_vals(const bws_bitmap* A, const void* B, int32_t precision, int64_t ldb,
@@ -279,18 +356,7 @@ bws_status spmm_mfma_vals(const bws_bitmap* A, const void* B, int32_t precision,
     hipLaunchKernelGGL(k_expand_values<1>, dim3((unsigned)g), dim3(256), 0, s, A->words, M, K,
                        A->ldw, A->row_ptr, woff, A->values, Ad, lda);
   BWS_TRY(launch_check());
-  const int64_t row_blocks = ceil_div(M, VBM), col_blocks = ceil_div(N, VBN);
-  const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
-  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
-  const dim3 grid((unsigned)(8 * per_xcd));
-  const uint16_t* Bh = static_cast<const uint16_t*>(B);
-  if (precision == BWS_PREC_FP16)
-    hipLaunchKernelGGL(k_spmm_mfma_vals<2>, grid, dim3(256), 0, s, Ad, M, K, lda, Bh, ldb, N, C,
-                       ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
-  else
-    hipLaunchKernelGGL(k_spmm_mfma_vals<1>, grid, dim3(256), 0, s, Ad, M, K, lda, Bh, ldb, N, C,
-                       ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
-  return launch_check();
+  return launch_vals_product(Ad, M, K, lda, B, precision, ldb, N, C, ldc, alpha, beta, s);
 }
 
 }  // namespace bws

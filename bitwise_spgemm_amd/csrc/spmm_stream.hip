@@ -1,0 +1,510 @@
+// spmm_stream.hip — FP32 SpMM of a 0/1 bitmap with the row masks streamed
+// through the scalar unit and B staged through a ring of LDS buffers (gfx950).
+//
+// C = alpha * A·B + beta * C: each set bit (r, k) adds row k of B to row r of C
+// ("replace the multiply with an add": the FP32 add path of
+// backup/layout_trans.cu:158-171 and the add-only loop of backup/sgemm.cu:428-436).
+//
+// Workgroup: 16 waves, 256 rows x 256 columns of C; a wave owns 16 rows (its
+// "chunk"), a lane 4 columns of each (16 float4 accumulators).
+//
+// B: K is walked in stages of KS rows.  A stage's KS x 256 panel of B goes
+// HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wave instruction, no
+// registers) into a ring of NB buffers, NB - 1 stages ahead of the stage being
+// summed.  Before a stage each wave waits only for its own copies of that
+// stage (s_waitcnt vmcnt with the younger stages' copies still in flight), then
+// one s_barrier makes every wave's copies visible and frees the buffer the
+// next copy overwrites.
+//
+// Masks: the bitmap is read as a mask stream — per chunk and stage, the
+// chunk's 16 row masks are contiguous (16 words, or 16 pairs for KS = 64) and
+// bit reversed (bit j of a stage mask <-> K row KS*stage + j).  A wave's masks
+// of a stage are one or two scalar loads (s_load_dwordx16) into SGPRs, loaded
+// one stage ahead; set bits are walked with s_ff1 / s_bitset0, so per set bit
+// the vector unit issues one LDS address and four adds, nothing else.
+//
+// Plain product: bws_mask_stream writes the stream from the bitmap (a pre-pass
+// per launch, 4 B read and 4 B written per word).  Row-clustered product: the
+// stream of the row-clustering plan (bws_cluster_rows) holds, per chunk and
+// stage, the slots' add masks (row & ~base for a delta row, the row
+// otherwise), the chunk's two base masks, a flag word (bit j: slot j has bits
+// to subtract in this stage) and the slots' subtract masks (base & ~row).
+#include "mfma_common.hpp"
+
+namespace bws {
+namespace {
+
+constexpr int kQW = 16;            // waves per workgroup
+constexpr int kQR = 16;            // rows (slots) per wave
+constexpr int kQRows = kQW * kQR;  // rows per workgroup
+constexpr int kQBN = 256;          // columns per workgroup (4 per lane)
+
+// Words of one chunk-stage record of the stream (MW = mask words per row).
+template <bool CL, int MW>
+struct Rec {
+  static constexpr int words = CL ? 48 * MW : 16 * MW;
+  static constexpr int base = 16 * MW;      // CL: base 0, base 1
+  static constexpr int flags = 18 * MW;     // CL: bit j = slot j subtracts
+  static constexpr int sub = 32 * MW;       // CL: subtract masks
+};
+
+typedef const __attribute__((address_space(4))) uint32_t* sgpr_src;
+
+template <int MW>
+struct Mask;
+template <>
+struct Mask<1> {
+  using T = uint32_t;
+  static __device__ __forceinline__ T load(sgpr_src p, int i) { return p[i]; }
+  static __device__ __forceinline__ int take(T& m) {
+    int k;
+    asm volatile("s_ff1_i32_b32 %0, %1\n\ts_bitset0_b32 %1, %0" : "=&s"(k), "+s"(m));
+    return k;
+  }
+  static __device__ __forceinline__ int count(T m) {
+    int n;
+    asm("s_bcnt1_i32_b32 %0, %1" : "=s"(n) : "s"(m));
+    return n;
+  }
+};
+template <>
+struct Mask<2> {
+  using T = uint64_t;
+  static __device__ __forceinline__ T load(sgpr_src p, int i) {
+    return ((uint64_t)p[2 * i + 1] << 32) | p[2 * i];
+  }
+  static __device__ __forceinline__ int take(T& m) {
+    int k;
+    asm volatile("s_ff1_i32_b64 %0, %1\n\ts_bitset0_b64 %1, %0" : "=&s"(k), "+s"(m));
+    return k;
+  }
+  static __device__ __forceinline__ int count(T m) {
+    int n;
+    asm("s_bcnt1_i32_b64 %0, %1" : "=s"(n) : "s"(m));
+    return n;
+  }
+};
+
+__device__ __forceinline__ f32x4 lds_row(uint32_t vbase, int k) {
+  return *reinterpret_cast<const __attribute__((address_space(3))) f32x4*>(
+      (uintptr_t)(vbase + ((uint32_t)k << 10)));
+}
+
+template <bool NEG, class V>
+__device__ __forceinline__ void acc_add(float4& a, const V& b) {
+  if (NEG) {
+    a.x -= b.x;
+    a.y -= b.y;
+    a.z -= b.z;
+    a.w -= b.w;
+  } else {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+    a.w += b.w;
+  }
+}
+
+// Adds (NEG: subtracts) the LDS rows of the set bits of m to acc.
+// The scalar unit issues every instruction of these walks for the 16 waves of
+// a CU, so the common cases are kept short: an empty mask costs a compare and
+// a branch, a one-bit mask the bit scan and one more compare and branch.  The
+// row's bits are summed into t first (ascending k, up to four LDS reads in
+// flight) and t is added to acc once: every path then ends in the same add, so
+// the accumulators keep their registers (no copies at the joins).
+template <int MW, bool NEG>
+__device__ __forceinline__ void walk(float4& acc, typename Mask<MW>::T m, uint32_t vbase) {
+  if (m != 0) {
+    f32x4 t = lds_row(vbase, Mask<MW>::take(m));
+    if (m != 0) {
+      const uint32_t n = (uint32_t)Mask<MW>::count(m);
+      if (n & 1u) t += lds_row(vbase, Mask<MW>::take(m));
+      if (n & 2u) {
+        const int k0 = Mask<MW>::take(m), k1 = Mask<MW>::take(m);
+        const f32x4 b0 = lds_row(vbase, k0), b1 = lds_row(vbase, k1);
+        t += b0;
+        t += b1;
+      }
+      for (uint32_t q = n >> 2; q != 0u; --q) {
+        const int k0 = Mask<MW>::take(m), k1 = Mask<MW>::take(m), k2 = Mask<MW>::take(m),
+                  k3 = Mask<MW>::take(m);
+        const f32x4 b0 = lds_row(vbase, k0), b1 = lds_row(vbase, k1), b2 = lds_row(vbase, k2),
+                    b3 = lds_row(vbase, k3);
+        t += b0;
+        t += b1;
+        t += b2;
+        t += b3;
+      }
+    }
+    acc_add<NEG>(acc, t);
+  }
+}
+
+__device__ __forceinline__ void s_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// s_waitcnt vmcnt(n * RPS) for a runtime n in [0, D)
+template <int RPS, int D>
+__device__ __forceinline__ void wait_younger(int n) {
+  if (D > 3 && n >= 3) {
+    wait_vmcnt<(D > 3 ? 3 : 0) * RPS>();
+  } else if (D > 2 && n >= 2) {
+    wait_vmcnt<(D > 2 ? 2 : 0) * RPS>();
+  } else if (D > 1 && n >= 1) {
+    wait_vmcnt<RPS>();
+  } else {
+    wait_vmcnt<0>();
+  }
+}
+
+template <int KS, int NB, bool CL>
+__global__ void __launch_bounds__(kQW * 64)
+k_spmm_stream(const uint32_t* __restrict__ S, const int32_t* __restrict__ slot_row, int64_t M,
+              int64_t nst, const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
+              float* __restrict__ C, int64_t ldc, float alpha, float beta, int64_t row_blocks,
+              int64_t col_blocks, int64_t per_xcd) {
+  static_assert(KS == 32 || KS == 64, "a stage is one or two mask words");
+  static_assert(NB >= 2 && NB <= 5 && NB * KS <= 160, "the ring must fit 160 KiB of LDS");
+  constexpr int MW = KS / 32;
+  constexpr int RPS = KS / kQW;  // copies per wave per stage (one B row each)
+  constexpr int D = NB - 1;      // stages in flight ahead of the one summed
+  using R = Rec<CL, MW>;
+  using MT = typename Mask<MW>::T;
+  constexpr bool TWO = MW == 1;
+  __shared__ float4 sB[NB * KS][64];
+
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
+  const int64_t n0 = cb * kQBN;
+  const int64_t chunk = rb * kQW + wave;
+  const int64_t ncol = (n0 + 4 * lane + 4 <= N) ? n0 + 4 * lane : N - 4;  // clamped column
+  const float* bcol = B + ncol;
+  const uint32_t lds0 = lds_u32(&sB[0][0]);
+  const sgpr_src rec = (sgpr_src)(S + chunk * nst * R::words);
+
+  // copy stage st's B rows of this wave into ring slot `slot`: B rows
+  // st*KS + wave + 16j, j < RPS (rows past K read row K-1, never summed)
+  const float* bw = bcol + (int64_t)wave * ldb;
+  const int64_t row_step = (int64_t)kQW * ldb;  // elements between a wave's rows
+  auto issue = [&](int64_t st, int slot) {
+    const uint32_t m0 = lds0 + (uint32_t)(slot * KS + wave) * 1024u;
+    if ((st + 1) * KS <= K) {
+      const float* p = bw + st * KS * ldb;
+#pragma unroll
+      for (int j = 0; j < RPS; ++j) dma16(p + j * row_step, m0 + (uint32_t)(j * kQW) * 1024u);
+    } else {
+#pragma unroll
+      for (int j = 0; j < RPS; ++j) {
+        int64_t k = st * KS + wave + kQW * j;
+        k = k < K ? k : K - 1;
+        dma16(bcol + k * ldb, m0 + (uint32_t)(j * kQW) * 1024u);
+      }
+    }
+  };
+
+  float4 acc[kQR];
+#pragma unroll
+  for (int r = 0; r < kQR; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 accT0 = make_float4(0.f, 0.f, 0.f, 0.f), accT1 = accT0;
+
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+    if (j < nst) issue(j, j);
+
+  // Mask registers.  A scalar load still in flight makes every LDS wait wait
+  // for it too (scalar loads return out of order), so a wave's scalar loads are
+  // issued once per stage, half way through its rows, when the waves of a SIMD
+  // are no longer in step.
+  //  TWO (KS = 32): two sets; the loads fill the other set with stage st + 1.
+  //  one set (KS = 64, 32 SGPRs of masks): rows 0..7 of stage st were loaded
+  //  half way through stage st - 1; half way through stage st come rows 8..15
+  //  (and the chunk words) of stage st and rows 0..7 of stage st + 1.
+  struct Set {
+    MT m[kQR];
+    MT b0, b1;
+    uint32_t fl;
+  };
+  auto load_rows = [&](Set& x, int64_t st, int r0, int r1) {
+    const sgpr_src p = rec + st * R::words;
+#pragma unroll
+    for (int r = r0; r < r1; ++r) x.m[r] = Mask<MW>::load(p, r);
+  };
+  auto load_chunk = [&](Set& x, int64_t st) {
+    if (CL) {
+      const sgpr_src p = rec + st * R::words;
+      x.b0 = Mask<MW>::load(p + R::base, 0);
+      x.b1 = Mask<MW>::load(p + R::base, 1);
+      x.fl = p[R::flags];
+    }
+  };
+  Set sa, sb;
+  load_rows(sa, 0, 0, TWO ? kQR : kQR / 2);
+  if (TWO) load_chunk(sa, 0);
+
+  int slot = 0;
+  auto stage = [&](int64_t st, Set& cur, Set& nxt) {
+    // this wave's copies of stage st have landed (younger stages may be in flight)
+    const int64_t younger = nst - 1 - st;
+    wait_younger<RPS, D>(younger < D - 1 ? (int)younger : D - 1);
+    s_barrier();  // every wave's copies of st landed; the slot of st - 1 is free
+    {
+      int ns = slot + D;
+      ns -= ns >= NB ? NB : 0;
+      if (st + D < nst) issue(st + D, ns);
+    }
+    const uint32_t vbase = lds0 + (uint32_t)slot * (KS * 1024u) + (uint32_t)lane * 16u;
+#pragma unroll
+    for (int r = 0; r < kQR; ++r) {
+      if (r == kQR / 2) {
+        if (TWO) {
+          if (st + 1 < nst) {
+            load_rows(nxt, st + 1, 0, kQR);
+            load_chunk(nxt, st + 1);
+          }
+        } else {
+          load_rows(cur, st, kQR / 2, kQR);
+          load_chunk(cur, st);
+          if (st + 1 < nst) load_rows(cur, st + 1, 0, kQR / 2);
+        }
+      }
+      walk<MW, false>(acc[r], cur.m[r], vbase);
+    }
+    if (CL) {
+      if (cur.b0) walk<MW, false>(accT0, cur.b0, vbase);
+      if (cur.b1) walk<MW, false>(accT1, cur.b1, vbase);
+      if (cur.fl) {
+        const sgpr_src p = rec + st * R::words + R::sub;
+#pragma unroll
+        for (int r = 0; r < kQR; ++r)
+          if ((cur.fl >> r) & 1u) {
+            MT sm = Mask<MW>::load(p, r);
+            walk<MW, true>(acc[r], sm, vbase);
+          }
+      }
+    }
+    slot = slot + 1 == NB ? 0 : slot + 1;
+  };
+  if (TWO) {
+    for (int64_t st = 0; st < nst; st += 2) {
+      stage(st, sa, sb);
+      if (st + 1 < nst) stage(st + 1, sb, sa);
+    }
+  } else {
+    for (int64_t st = 0; st < nst; ++st) stage(st, sa, sa);
+  }
+
+  // slot rows (CL) and which slots add a base, taken before any lane leaves
+  int32_t srow = -1;
+  if (CL) {
+    const int64_t s0 = chunk * kQR;
+    if (lane < kQR && s0 + lane < M) srow = slot_row[s0 + lane];
+  }
+  const bool d = lane < kQR && srow >= 0 && (srow & BWS_SLOT_DELTA);
+  const uint32_t use0 = CL ? (uint32_t)__ballot(d && !(srow & BWS_SLOT_BASE1)) : 0u;
+  const uint32_t use1 = CL ? (uint32_t)__ballot(d && (srow & BWS_SLOT_BASE1)) : 0u;
+  const int64_t n = n0 + 4 * lane;
+  if (n >= N) return;
+#pragma unroll
+  for (int r = 0; r < kQR; ++r) {
+    int64_t row;
+    float4 v = acc[r];
+    if (CL) {
+      const int32_t sr = __builtin_amdgcn_readlane(srow, r);
+      if (sr < 0) continue;
+      row = sr & (BWS_SLOT_BASE1 - 1);
+      if ((use0 >> r) & 1u) acc_add<false>(v, accT0);
+      if ((use1 >> r) & 1u) acc_add<false>(v, accT1);
+    } else {
+      row = chunk * kQR + r;
+      if (row >= M) break;
+    }
+    float4* c = reinterpret_cast<float4*>(C + row * ldc + n);
+    v.x *= alpha;
+    v.y *= alpha;
+    v.z *= alpha;
+    v.w *= alpha;
+    if (beta != 0.0f) {
+      const float4 o = *c;
+      v.x = fmaf(beta, o.x, v.x);
+      v.y = fmaf(beta, o.y, v.y);
+      v.z = fmaf(beta, o.z, v.z);
+      v.w = fmaf(beta, o.w, v.w);
+    }
+    *c = v;
+  }
+}
+
+// Bits of word w (LSB-first after reversal) that are columns < cols.
+__device__ __forceinline__ uint32_t valid_lsb(int64_t w, int64_t cols) {
+  const int64_t rem = cols - 32 * w;
+  return rem >= 32 ? 0xFFFFFFFFu : (rem <= 0 ? 0u : ((1u << rem) - 1u));
+}
+
+// Mask stream of a plain bitmap: a thread per (row, group of 8 stages).  Rows
+// beyond M and bits beyond K are written as zero.
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_mask_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw, int64_t nst,
+              uint32_t* __restrict__ S) {
+  const int t = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.y * kQRows + t;
+  const int64_t chunk = r / kQR;
+  const int in = t % kQR;
+  const int64_t wpr = words_for(K);
+  for (int j = 0; j < 8; ++j) {
+    const int64_t st = (int64_t)blockIdx.x * 8 + j;
+    if (st >= nst) break;
+    uint32_t* out = S + (chunk * nst + st) * (16 * MW) + in * MW;
+#pragma unroll
+    for (int h = 0; h < MW; ++h) {
+      const int64_t w = st * MW + h;
+      uint32_t x = 0;
+      if (r < M && w < wpr) x = __brev(W[r * ldw + w]) & valid_lsb(w, K);
+      out[h] = x;
+    }
+  }
+}
+
+// Mask stream of a row-clustering plan: a thread per (chunk, stage).
+template <int MW>
+__global__ void __launch_bounds__(256)
+k_cluster_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+                 const int32_t* __restrict__ slot_row, const uint32_t* __restrict__ CM,
+                 int64_t chunks, int64_t nst, uint32_t* __restrict__ S) {
+  using R = Rec<true, MW>;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= chunks * nst) return;
+  const int64_t c = t / nst, st = t - c * nst;
+  const int64_t wpr = words_for(K);
+  const bool real = c * kQR < M;  // chunks past the last row block's end are all zero
+  uint32_t* out = S + t * R::words;
+  // which bases some slot of the chunk uses
+  bool used0 = false, used1 = false;
+  int32_t sr[kQR];
+#pragma unroll
+  for (int j = 0; j < kQR; ++j) {
+    const int64_t s = c * kQR + j;
+    sr[j] = s < M ? slot_row[s] : -1;
+    if (sr[j] >= 0 && (sr[j] & BWS_SLOT_DELTA)) {
+      if (sr[j] & BWS_SLOT_BASE1) used1 = true;
+      else used0 = true;
+    }
+  }
+  uint32_t flags = 0;
+#pragma unroll
+  for (int h = 0; h < MW; ++h) {
+    const int64_t w = st * MW + h;
+    const bool ok = real && w < wpr;
+    const uint32_t vb = ok ? valid_lsb(w, K) : 0u;
+    const uint32_t b0 = ok ? __brev(CM[(2 * c) * ldw + w]) & vb : 0u;
+    const uint32_t b1 = ok ? __brev(CM[(2 * c + 1) * ldw + w]) & vb : 0u;
+    out[R::base + h] = used0 ? b0 : 0u;
+    out[R::base + MW + h] = used1 ? b1 : 0u;
+#pragma unroll
+    for (int j = 0; j < kQR; ++j) {
+      uint32_t add = 0, sub = 0;
+      if (sr[j] >= 0 && ok) {
+        const uint32_t x = __brev(W[(int64_t)(sr[j] & (BWS_SLOT_BASE1 - 1)) * ldw + w]) & vb;
+        if (sr[j] & BWS_SLOT_DELTA) {
+          const uint32_t b = (sr[j] & BWS_SLOT_BASE1) ? b1 : b0;
+          add = x & ~b;
+          sub = b & ~x;
+        } else {
+          add = x;
+        }
+      }
+      out[j * MW + h] = add;
+      out[R::sub + j * MW + h] = sub;
+      if (sub) flags |= 1u << j;
+    }
+  }
+  out[R::flags] = flags;
+#pragma unroll
+  for (int i = R::flags + 1; i < R::sub; ++i) out[i] = 0u;
+}
+
+template <int KS>
+size_t mask_stream_bytes_t(int64_t M, int64_t K) {
+  return (size_t)ceil_div(M, kQRows) * kQW * ceil_div(K, KS) * Rec<false, KS / 32>::words * 4;
+}
+// The cluster stream covers whole row blocks (16 chunks each), as the product
+// reads them; chunks past the rows are zero.
+template <int KS>
+size_t cluster_stream_bytes_t(int64_t M, int64_t K) {
+  return (size_t)ceil_div(M, kQRows) * kQW * ceil_div(K, KS) * Rec<true, KS / 32>::words * 4;
+}
+
+template <int KS>
+bws_status launch_mask_stream_t(const bws_bitmap* A, uint32_t* S, hipStream_t s) {
+  const int64_t nst = ceil_div(A->cols, KS);
+  const int64_t rbs = ceil_div(A->rows, kQRows);
+  if (rbs > 65535 || ceil_div(nst, 8) > kMaxGrid) return BWS_ECAPACITY;
+  hipLaunchKernelGGL(k_mask_stream<KS / 32>, dim3((unsigned)ceil_div(nst, 8), (unsigned)rbs),
+                     dim3(256), 0, s, A->words, A->rows, A->cols, A->ldw, nst, S);
+  return launch_check();
+}
+
+template <int KS>
+bws_status launch_cluster_stream_t(const bws_bitmap* A, const int32_t* slot_row,
+                                   const uint32_t* chunk_mask, uint32_t* S, hipStream_t s) {
+  const int64_t nst = ceil_div(A->cols, KS);
+  const int64_t chunks = ceil_div(A->rows, kQRows) * kQW;
+  const int64_t n = chunks * nst;
+  if (ceil_div(n, 256) > kMaxGrid) return BWS_ECAPACITY;
+  hipLaunchKernelGGL(k_cluster_stream<KS / 32>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                     A->words, A->rows, A->cols, A->ldw, slot_row, chunk_mask, chunks, nst, S);
+  return launch_check();
+}
+
+// The streamed FP32 product over a stream of launch_mask_stream_t (slot_row
+// NULL) or launch_cluster_stream_t (slot_row = the plan's).  N, ldb and ldc
+// multiples of 4 and B, C 16-byte aligned (checked by the callers).
+template <int KS, int NB>
+bws_status launch_spmm_stream_t(const uint32_t* S, const int32_t* slot_row, int64_t M, int64_t K,
+                                const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
+                                float alpha, float beta, hipStream_t s) {
+  const int64_t nst = ceil_div(K, KS);
+  const int64_t row_blocks = ceil_div(M, kQRows);
+  const int64_t col_blocks = ceil_div(N, kQBN);
+  const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  if (slot_row)
+    hipLaunchKernelGGL((k_spmm_stream<KS, NB, true>), dim3((unsigned)(8 * per_xcd)),
+                       dim3(kQW * 64), 0, s, S, slot_row, M, nst, B, ldb, K, N, C, ldc, alpha,
+                       beta, row_blocks, col_blocks, per_xcd);
+  else
+    hipLaunchKernelGGL((k_spmm_stream<KS, NB, false>), dim3((unsigned)(8 * per_xcd)),
+                       dim3(kQW * 64), 0, s, S, slot_row, M, nst, B, ldb, K, N, C, ldc, alpha,
+                       beta, row_blocks, col_blocks, per_xcd);
+  return launch_check();
+}
+
+}  // namespace
+
+#ifndef BWS_STREAM_NO_PRODUCT
+// The configuration the library runs: K rows per stage and LDS ring buffers.
+constexpr int kStreamKS = 64;
+constexpr int kStreamNB = 2;
+
+size_t mask_stream_bytes(int64_t M, int64_t K) { return mask_stream_bytes_t<kStreamKS>(M, K); }
+size_t cluster_stream_bytes(int64_t M, int64_t K) {
+  return cluster_stream_bytes_t<kStreamKS>(M, K);
+}
+bws_status launch_mask_stream(const bws_bitmap* A, uint32_t* S, hipStream_t s) {
+  return launch_mask_stream_t<kStreamKS>(A, S, s);
+}
+bws_status launch_cluster_stream(const bws_bitmap* A, const int32_t* slot_row,
+                                 const uint32_t* chunk_mask, uint32_t* S, hipStream_t s) {
+  return launch_cluster_stream_t<kStreamKS>(A, slot_row, chunk_mask, S, s);
+}
+bws_status launch_spmm_stream(const uint32_t* S, const int32_t* slot_row, int64_t M, int64_t K,
+                              const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
+                              float alpha, float beta, hipStream_t s) {
+  return launch_spmm_stream_t<kStreamKS, kStreamNB>(S, slot_row, M, K, B, ldb, N, C, ldc, alpha,
+                                                    beta, s);
+}
+#endif
+
+}  // namespace bws

@@ -25,8 +25,7 @@
  *     BWS_EWORKSPACE otherwise).  All work is enqueued on the caller's stream with
  *     no host synchronisation, so the entry points can be captured into a
  *     hipGraph, except the set-up calls bws_batch_prepare and bws_autotune
- *     (they copy or time on the host) and bws_cluster_rows (its row sort may
- *     record memset nodes).
+ *     (they copy or time on the host).
  *   - Every entry point returns a bws_status (0 = BWS_OK, negative = error); no
  *     entry point aborts or prints.  Argument validation happens on the host
  *     before any HIP call, so invalid calls are rejected without a GPU.
@@ -276,10 +275,14 @@ bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64
  * of backup/exp.cu:49-88); this analysis step has no reference counterpart:
  * it orders the rows so that each run of 16 slots holds similar masks, and
  * bws_spmm_clustered sums each chunk base's bits once.
- *   leaders: farthest point over Hamming distance.  Leader 0 is row 0; every
- *     row keeps its nearest leader (the earliest on ties); leader i+1 is the
- *     row farthest from its nearest leader (the lowest row on ties);
- *   order:   rows by nearest leader, ascending within a leader;
+ *   leaders: L = max_clusters (0: rows / 64), clamped to [1, min(rows, 65536)].
+ *     Candidates: c = min(rows, 256) rows when L <= 256, else c = L; candidate
+ *     j is row floor(j * rows / c).  L <= 256: farthest point over the
+ *     candidates with the distance min(Hamming, 65535): leader 0 is candidate
+ *     0, leader i+1 the candidate farthest from its nearest leader so far (the
+ *     lowest candidate on ties).  L > 256: leader i is candidate i;
+ *   order:   every row's nearest leader by Hamming distance (the earliest
+ *            leader on ties); rows by nearest leader, ascending within a leader;
  *   chunks:  chunk c is slots 16c .. 16c+15 (the last may hold fewer); its
  *            slots form runs of equal leader.  Base 0 comes from the longest
  *            run, base 1 from the second longest (the earlier run on equal
@@ -289,13 +292,14 @@ bws_status bws_group_gather(const float* D, int64_t ld, const uint32_t* W, int64
  *   slots:   slot_row[s] = row of slot s, plus BWS_SLOT_DELTA when some base
  *            has popcount(row ^ base) < popcount(row) (the smallest wins, base
  *            0 on ties), plus BWS_SLOT_BASE1 when that base is base 1.
- * max_clusters = 0 uses rows / 64 leaders; the count is clamped to
- * [1, min(rows, 65536)].  slot_row has `rows` entries, chunk_mask
- * ceil(rows/16) x 2 x A->ldw words.  0/1 masks only (A->values must be
- * NULL); rows < 2^29.  Deterministic: the same bitmap gives the same plan. */
+ * slot_row has `rows` entries, chunk_mask ceil(rows/16) x 2 x A->ldw words.
+ * 0/1 masks only (A->values must be NULL); rows < 2^29 and
+ * L * ceil(rows / 1024) <= 2^27 (BWS_ECAPACITY otherwise).  Deterministic, all
+ * on the device with no host synchronisation and no memset: capturable into a
+ * hipGraph. */
 #define BWS_SLOT_DELTA (1 << 30)
 #define BWS_SLOT_BASE1 (1 << 29)
-size_t bws_cluster_workspace_size(int64_t rows, int32_t max_clusters);
+size_t bws_cluster_workspace_size(int64_t rows, int64_t cols, int32_t max_clusters);
 bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* slot_row,
                             uint32_t* chunk_mask, void* workspace, size_t workspace_bytes,
                             bws_stream stream);
@@ -345,6 +349,13 @@ typedef struct {
  * BF16 with BF16 or F32 (rounded to nearest even); FP16 with F16 or F32;
  * BF16X2 with F32.
  * Workspace (optional; size from bws_spmm_workspace_size, 0 when unused):
+ *   - FP32 with a 0/1 A and N % 4 == 0: the mask stream of A (A's words bit
+ *     reversed and regrouped so that a wave's 16 row masks of a 64-row stage
+ *     are contiguous), written by a pass per call; the product then reads the
+ *     masks with scalar loads and walks their bits on the scalar unit (used
+ *     when B and C rows are 16-byte aligned and the problem has at least 192
+ *     tiles of 256 x 256; without the workspace the vector-add tile kernel
+ *     runs, with the same FP32 semantics);
  *   - value-carrying A at BF16/FP16 with 16-bit B: a dense 16-bit copy of A
  *     for the MFMA kernel (without it the FP32 vector path runs);
  *   - BF16X2 with a 0/1 A and N % 8 == 0: B's hi and lo BF16 planes, split
@@ -376,16 +387,31 @@ bws_status bws_spmm_groups(const bws_bitmap* A, const int32_t* group_id,
                            const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
                            bws_stream stream);
 
+/* The row-clustered product's operand stream, built once per plan: for every
+ * chunk of 16 slots and every 64-row stage of K, the slots' add masks
+ * (row & ~base for a delta row, the row itself otherwise), the chunk's two
+ * base masks (zero when no slot uses them), a flag word (bit j: slot j has
+ * bits to subtract in this stage) and the slots' subtract masks (base & ~row),
+ * all bit reversed (bit i <-> K row 64*stage + i) and laid out so that a
+ * wave's masks of a stage are contiguous.  bws_cluster_stream_size gives the
+ * bytes for a rows x cols bitmap (chunks padded to whole 256-row blocks).
+ * Capturable (device only). */
+size_t bws_cluster_stream_size(int64_t rows, int64_t cols);
+bws_status bws_cluster_stream(const bws_bitmap* A, const int32_t* slot_row,
+                              const uint32_t* chunk_mask, void* plan_stream, size_t stream_bytes,
+                              bws_stream stream);
+
 /* C = alpha * (A · B) + beta * C for a 0/1 bitmap A, FP32 B and C, with the
- * plan of bws_cluster_rows for this A: a wave takes one chunk; each delta row
- * is accumulated as (row & ~base)·B - (base & ~row)·B and its base's base·B,
- * summed once per chunk, is added at the end; other rows sum their own bits.
- * FP32 vector adds; the summation order differs from bws_spmm's, and the
- * result stays within FP32 rounding of A·B relative to |A|·|B| (the
- * subtracted terms are bits of the base).  Needs N, ldb and ldc multiples of
- * 4 and 16-byte aligned B and C (BWS_EUNSUPPORTED otherwise). */
+ * plan of bws_cluster_rows for this A and its stream from bws_cluster_stream:
+ * a wave takes one chunk; each delta row is accumulated as (row & ~base)·B -
+ * (base & ~row)·B and its base's base·B, summed once per chunk, is added at
+ * the end; other rows sum their own bits.  FP32 vector adds; the summation
+ * order differs from bws_spmm's, and the result stays within FP32 rounding of
+ * A·B relative to |A|·|B| (the subtracted terms are bits of the base).  Needs
+ * N, ldb and ldc multiples of 4 and 16-byte aligned B and C
+ * (BWS_EUNSUPPORTED otherwise). */
 bws_status bws_spmm_clustered(const bws_bitmap* A, const int32_t* slot_row,
-                              const uint32_t* chunk_mask, const float* B, int64_t ldb, int64_t N,
+                              const void* plan_stream, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, float alpha, float beta, bws_stream stream);
 
 /* Adjoint dB = alpha * Aᵀ·dC + beta * dB given AT = transpose of A (from
@@ -502,15 +528,40 @@ bws_status bws_symbolic_or(const uint32_t* WA, int64_t M, int64_t K, int64_t ldw
                            const uint32_t* WB, int64_t N, int64_t ldwb, uint32_t* WC,
                            int64_t ldwc, bws_stream stream);
 
+/* Numeric sparse x sparse product onto an output pattern: for the j-th set
+ * bit c of row r of C (C->row_ptr required), C_values[C->row_ptr[r] + j] =
+ * alpha * Σ_k A[r, k]·B[k, c], A and B bitmaps (value-carrying, or 0/1 when
+ * their values are NULL).  With C = bws_symbolic_or(A, B) this is the
+ * numeric phase of the reference's sparse x sparse pipeline
+ * (spgemm_compute_1dthread_tcore_v3, source/src/transform.cuh:2199-2246);
+ * bws_group_rows on the result (C with these values) gives its grouped form,
+ * the group values the reference stores in dC_group_value
+ * (source/src/main.cu:2133).  FP32 FMA in ascending k for every entry.
+ * Workspace: per-word popcount offsets of B and C. */
+size_t bws_spgemm_workspace_size(int64_t M, int64_t K, int64_t N);
+bws_status bws_spgemm(const bws_bitmap* A, const bws_bitmap* B, const bws_bitmap* C, float alpha,
+                      float* C_values, void* workspace, size_t workspace_bytes,
+                      bws_stream stream);
+
 /* int8-valued A held as 8 packed bit-planes P (bws_pack_bitplanes_i8):
  * C = alpha * Σ_b w_b (P_b · B) + beta * C with w = [1,2,4,...,64,-128], the
  * reference's plane recombination (source/src/main.cu:889-912,
- * transform.cuh:2245) run as eight 0/1 products.  FP32 rounding is bounded by
- * Σ_b |w_b| (P_b |B|), not by |A||B|. */
+ * transform.cuh:2245).  With the workspace (bws_spmm_bitplanes_workspace_size)
+ * and a NULL plan (or kernel 0) B is read once:
+ *   FP32 with FP32 B: the planes become a value-carrying bitmap of the int8
+ *     values (their OR as the mask), then one FP32 vector pass (FMA of each
+ *     exact int8 value, FP32 accumulation: rounding bounded by |A|·|B|);
+ *   BF16 / FP16 with 16-bit B (16-byte rows, N % 8 == 0): the planes are
+ *     folded into a dense 16-bit copy of A (int8 values are exact in both),
+ *     then one MFMA product;
+ * otherwise eight weighted 0/1 products (FP32 rounding bounded by
+ * Σ_b |w_b| (P_b |B|)). */
+size_t bws_spmm_bitplanes_workspace_size(int64_t M, int64_t K, int32_t precision);
 bws_status bws_spmm_bitplanes_i8(const uint32_t* P, int64_t M, int64_t K, int64_t ldw,
                                  const void* B, int32_t b_dtype, int64_t ldb, int64_t N, float* C,
                                  int64_t ldc, float alpha, float beta, int32_t precision,
-                                 const bws_plan* plan, bws_stream stream);
+                                 const bws_plan* plan, void* workspace, size_t workspace_bytes,
+                                 bws_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side deterministic generators (no GPU; xoshiro256** seeded by        */

@@ -42,6 +42,8 @@ _SIGS = {
     "oref_group_gather": [_P, _I64, _P, _I64, _I64, _I64, _P, _I32, _I32, _P],
     "oref_symbolic_or": [_P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64],
     "oref_spmm_f64": [_P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _P, _I64, _I32],
+    "oref_spgemm_f64": [_P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _I64, _P, _P,
+                        _I32],
     "oref_spmm_f32_seq": [_P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _P, _I64],
     "oref_spmm_groups_f64": [_P, _I64, _I64, _I64, _P, _P, _P, _P, _I32, _I32, _P, _I64,
                              _I64, _P, _I64],
@@ -49,12 +51,15 @@ _SIGS = {
     "oref_check_dense_f32": [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I32],
     "oref_err_metrics": [_P, _I64, _P, _P, _I64, _I64, _P],
     "oref_cluster_leaders": [_I64, _I32],
+    "oref_cluster_candidates": [_I64, _I32],
+    "oref_cluster_exact_rounds": [_I32],
     "oref_cluster_rows": [_P, _I64, _I64, _I64, _I32, _P, _P],
     "oref_round_bf16_array": [_P, _I64, _P],
     "oref_round_f16_array": [_P, _I64, _P],
     "oref_split_bf16x2": [_P, _I64, _P, _P],
 }
-_RET = {"oref_group_tiles": _I64, "oref_cluster_leaders": _I32}
+_RET = {"oref_group_tiles": _I64, "oref_cluster_leaders": _I32,
+        "oref_cluster_candidates": _I64, "oref_cluster_exact_rounds": _I32}
 
 
 def build() -> str:
@@ -222,6 +227,14 @@ def cluster_leaders(rows, max_clusters=0):
     return int(lib().oref_cluster_leaders(rows, max_clusters))
 
 
+def cluster_exact_rounds(L):
+    return int(lib().oref_cluster_exact_rounds(L))
+
+
+def cluster_candidates(rows, L):
+    return int(lib().oref_cluster_candidates(rows, L))
+
+
 def cluster_rows(W, cols, max_clusters=0):
     """Row-clustering plan: (slot_row int32 [rows], chunk_mask uint32
     [ceil(rows/16), 2, ldw] (two bases per chunk))."""
@@ -302,6 +315,21 @@ def spmm_f64(W, K, B, rp=None, vals=None, absolute=False):
                         _p(None if vals is None else _c(vals, np.float32)), _p(B), N, N,
                         _p(Cm), N, 1 if absolute else 0)
     return Cm
+
+
+def spgemm_f64(WA, K, WB, N, WC, rpC, rpA=None, vA=None, rpB=None, vB=None, absolute=False):
+    """A·B (both bitmaps, value-carrying when their values are given) at the
+    set bits of WC, in WC's row-major bit order (FP64)."""
+    WA, WB, WC = _c(WA, np.uint32), _c(WB, np.uint32), _c(WC, np.uint32)
+    M = WA.shape[0]
+    out = np.empty(int(rpC[-1]), np.float64)
+    lib().oref_spgemm_f64(_p(WA), M, K, WA.shape[1],
+                          _p(None if rpA is None else _c(rpA, np.int64)),
+                          _p(None if vA is None else _c(vA, np.float32)), _p(WB), N, WB.shape[1],
+                          _p(None if rpB is None else _c(rpB, np.int64)),
+                          _p(None if vB is None else _c(vB, np.float32)), _p(WC), WC.shape[1],
+                          _p(_c(rpC, np.int64)), _p(out), 1 if absolute else 0)
+    return out
 
 
 def spmm_f32_seq(W, K, B, rp=None, vals=None):

@@ -344,16 +344,38 @@ int32_t oref_cluster_leaders(int64_t rows, int32_t max_clusters) {
   return (int32_t)L;
 }
 
+/* Leaders taken by the exact farthest-point rounds over all rows: the last
+ * min(8, L - 1) of L. */
+int32_t oref_cluster_exact_rounds(int32_t L) { return L - 1 < 8 ? (L > 0 ? L - 1 : 0) : 8; }
+
+/* Candidate rows of the leader search for the first Lc = L - exact rounds
+ * leaders: Lc <= 256 searches min(rows, 256) candidates; more take the
+ * candidates themselves (c = Lc).  Candidate j is row floor(j * rows / c). */
+int64_t oref_cluster_candidates(int64_t rows, int32_t L) {
+  if (rows <= 0) return 0;
+  const int32_t Lc = L - oref_cluster_exact_rounds(L);
+  if (Lc > 256) return Lc;
+  return rows < 256 ? rows : 256;
+}
+
 /* The row-clustering plan, serially.  There is no reference function to
  * follow: the reference groups rows by bitmask (generate_groups_large,
  * source/src/main.cu:50-247; the row-to-group intent of backup/exp.cu:49-88)
  * so that a group's rows share work; this plan groups similar rows so that a
  * chunk's shared bits are summed once.
- *   1. Farthest-point leaders over Hamming distance: leader 0 is row 0; after
- *      every row has taken its distance to leader i (keeping the earliest
- *      leader on ties), leader i+1 is the row with the largest distance to
- *      its nearest leader (lowest row on ties).
- *   2. Rows ordered by nearest leader, ascending row within a leader.
+ *   1. Leaders.  R = min(8, L - 1) exact rounds, Lc = L - R.  c =
+ *      oref_cluster_candidates(rows, L) candidate rows, candidate j = row
+ *      floor(j * rows / c).  Leaders 0 .. Lc-1: with Lc <= 256, farthest
+ *      point over the candidates with the distance min(popcount(x ^ y),
+ *      65535): leader 0 is candidate 0; every candidate keeps its distance to
+ *      the nearest leader chosen so far; the next leader is the candidate with
+ *      the largest such distance (the lowest candidate on ties); with Lc > 256,
+ *      leader i is candidate i.  Leaders Lc .. L-1: farthest point over all
+ *      rows with the full Hamming distance: the row whose distance to its
+ *      nearest leader so far is largest (the lowest row on ties).
+ *   2. Every row's nearest leader by full Hamming distance (the earliest
+ *      leader on ties); rows ordered by nearest leader, ascending row within
+ *      a leader.
  *   3. Chunk c = slots 16c .. 16c+15 (fewer in the last).  Its rows form runs
  *      of equal leader; base 0 is made from the longest run, base 1 from the
  *      second longest (earlier run first on equal lengths; no base 1 when the
@@ -363,32 +385,70 @@ int32_t oref_cluster_leaders(int64_t rows, int32_t max_clusters) {
  *      difference from a base: the base j minimising popcount(row ^ base_j),
  *      if that is below popcount(row) (base 0 on ties); (1 << 29) marks
  *      base 1. */
+static int64_t hamming_rows(const uint32_t* W, int64_t ldw, int64_t cols, int64_t a, int64_t b) {
+  const int64_t wpr = words_for(cols);
+  int64_t d = 0;
+  for (int64_t w = 0; w < wpr; ++w)
+    d += popc32((W[a * ldw + w] ^ W[b * ldw + w]) & valid_word(w, cols));
+  return d;
+}
+
 void oref_cluster_rows(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
                        int32_t max_clusters, int32_t* slot_row, uint32_t* chunk_mask) {
   if (rows <= 0) return;
   const int64_t wpr = words_for(cols);
   const int32_t L = oref_cluster_leaders(rows, max_clusters);
-  int64_t* dist = (int64_t*)malloc(sizeof(int64_t) * rows);
+  const int64_t c = oref_cluster_candidates(rows, L);
+  int64_t* lead_row = (int64_t*)malloc(sizeof(int64_t) * L);
   int32_t* owner = (int32_t*)malloc(sizeof(int32_t) * rows);
   int64_t* start = (int64_t*)calloc((size_t)L + 1, sizeof(int64_t));
   int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * rows);
-  int64_t lead = 0;
-  for (int32_t it = 0; it < L; ++it) {
-    const uint32_t* lw = W + lead * ldw;
-    for (int64_t r = 0; r < rows; ++r) {
-      int64_t d = 0;
-      for (int64_t w = 0; w < wpr; ++w)
-        d += popc32((W[r * ldw + w] ^ lw[w]) & valid_word(w, cols));
-      if (it == 0 || d < dist[r]) {
+  const int32_t Lc = L - oref_cluster_exact_rounds(L);
+  if (Lc > 256) {
+    for (int32_t i = 0; i < Lc; ++i) lead_row[i] = (int64_t)i * rows / c;
+  } else {
+    int64_t* near = (int64_t*)malloc(sizeof(int64_t) * c);
+    int64_t lead = 0;
+    for (int32_t it = 0; it < Lc; ++it) {
+      lead_row[it] = lead * rows / c;
+      for (int64_t j = 0; j < c; ++j) {
+        int64_t d = hamming_rows(W, ldw, cols, j * rows / c, lead_row[it]);
+        if (d > 65535) d = 65535;
+        if (it == 0 || d < near[j]) near[j] = d;
+      }
+      int64_t best = 0;
+      for (int64_t j = 1; j < c; ++j)
+        if (near[j] > near[best]) best = j;
+      lead = best;
+    }
+    free(near);
+  }
+  /* nearest of the first Lc leaders, then the exact rounds */
+  int64_t* dist = (int64_t*)malloc(sizeof(int64_t) * rows);
+  for (int64_t r = 0; r < rows; ++r) {
+    dist[r] = -1;
+    for (int32_t i = 0; i < Lc; ++i) {
+      const int64_t d = hamming_rows(W, ldw, cols, r, lead_row[i]);
+      if (dist[r] < 0 || d < dist[r]) {
         dist[r] = d;
-        owner[r] = it;
+        owner[r] = i;
       }
     }
+  }
+  for (int32_t i = Lc; i < L; ++i) {
     int64_t best = 0;
     for (int64_t r = 1; r < rows; ++r)
       if (dist[r] > dist[best]) best = r;
-    lead = best;
+    lead_row[i] = best;
+    for (int64_t r = 0; r < rows; ++r) {
+      const int64_t d = hamming_rows(W, ldw, cols, r, best);
+      if (d < dist[r]) {
+        dist[r] = d;
+        owner[r] = i;
+      }
+    }
   }
+  free(dist);
   for (int64_t r = 0; r < rows; ++r) start[owner[r] + 1]++;
   for (int32_t o = 0; o < L; ++o) start[o + 1] += start[o];
   for (int64_t r = 0; r < rows; ++r) perm[start[owner[r]]++] = (int32_t)r;
@@ -441,7 +501,7 @@ void oref_cluster_rows(const uint32_t* W, int64_t rows, int64_t cols, int64_t ld
     if (p1 < best) flag = (1 << 30) | (1 << 29);
     slot_row[s] = (int32_t)r | flag;
   }
-  free(dist);
+  free(lead_row);
   free(owner);
   free(start);
   free(perm);
@@ -478,6 +538,45 @@ void oref_spmm_f64(const uint32_t* W, int64_t M, int64_t K, int64_t ldw, const i
       }
     }
   }
+}
+
+/* Numeric sparse x sparse product onto an output pattern: for the j-th set
+ * bit c of row r of WC, out[rpC[r] + j] = Σ_k A[r, k]·B[k, c] in FP64 (A, B
+ * value-carrying bitmaps, or 0/1 when their values are NULL; absolute: Σ
+ * |A|·|B|).  The numeric phase of the reference's sparse x sparse pipeline
+ * (spgemm_compute_1dthread_tcore_v3, source/src/transform.cuh:2199-2246,
+ * whose per-group sums land in dC_group_value, source/src/main.cu:2133) on
+ * the pattern its symbolic phase builds (transform.cuh:1028-1061). */
+void oref_spgemm_f64(const uint32_t* WA, int64_t M, int64_t K, int64_t ldwa, const int64_t* rpA,
+                     const float* vA, const uint32_t* WB, int64_t N, int64_t ldwb,
+                     const int64_t* rpB, const float* vB, const uint32_t* WC, int64_t ldwc,
+                     const int64_t* rpC, double* out, int32_t absolute) {
+  const int64_t wpa = words_for(K), wpn = words_for(N);
+  double* row = (double*)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+  for (int64_t r = 0; r < M; ++r) {
+    for (int64_t n = 0; n < N; ++n) row[n] = 0.0;
+    int64_t ea = vA ? rpA[r] : 0;
+    for (int64_t x = 0; x < wpa; ++x)
+      for (int i = 0; i < 32; ++i) {
+        if (!(WA[r * ldwa + x] & valid_word(x, K) & WBIT(i))) continue;
+        const int64_t k = 32 * x + i;
+        double a = vA ? (double)vA[ea++] : 1.0;
+        if (absolute) a = fabs(a);
+        int64_t eb = vB ? rpB[k] : 0;
+        for (int64_t w = 0; w < wpn; ++w)
+          for (int j = 0; j < 32; ++j) {
+            if (!(WB[k * ldwb + w] & valid_word(w, N) & WBIT(j))) continue;
+            double b = vB ? (double)vB[eb++] : 1.0;
+            if (absolute) b = fabs(b);
+            row[32 * w + j] += a * b;
+          }
+      }
+    int64_t ec = rpC[r];
+    for (int64_t w = 0; w < wpn; ++w)
+      for (int j = 0; j < 32; ++j)
+        if (WC[r * ldwc + w] & valid_word(w, N) & WBIT(j)) out[ec++] = row[32 * w + j];
+  }
+  free(row);
 }
 
 /* FP32 in ascending k: acc = fl(acc + fl(a*b)), starting at 0 — the order of

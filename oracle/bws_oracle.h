@@ -65,10 +65,16 @@ void oref_symbolic_or(const uint32_t* WA, int64_t M, int64_t K, int64_t ldwa,
 /* --- row clustering (an analysis step of this library; no reference
  *     counterpart beyond the intent of grouping rows by bitmask) ---------- */
 int32_t oref_cluster_leaders(int64_t rows, int32_t max_clusters);
+int64_t oref_cluster_candidates(int64_t rows, int32_t L);
+int32_t oref_cluster_exact_rounds(int32_t L);
 void oref_cluster_rows(const uint32_t* W, int64_t rows, int64_t cols, int64_t ldw,
                        int32_t max_clusters, int32_t* slot_row, uint32_t* chunk_mask);
 
 /* --- numerics ---------------------------------------------------------- */
+void oref_spgemm_f64(const uint32_t* WA, int64_t M, int64_t K, int64_t ldwa, const int64_t* rpA,
+                     const float* vA, const uint32_t* WB, int64_t N, int64_t ldwb,
+                     const int64_t* rpB, const float* vB, const uint32_t* WC, int64_t ldwc,
+                     const int64_t* rpC, double* out, int32_t absolute);
 void oref_spmm_f64(const uint32_t* W, int64_t M, int64_t K, int64_t ldw, const int64_t* row_ptr,
                    const float* values, const float* B, int64_t ldb, int64_t N, double* C,
                    int64_t ldc, int32_t absolute);

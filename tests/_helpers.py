@@ -40,3 +40,24 @@ def rel_err(oracle, C, W, K, B, rp=None, vals=None, B_exact=None):
     R = oracle.spmm_f64(W, K, Bref, rp, vals)
     S = oracle.spmm_f64(W, K, Bref, rp, vals, absolute=True)
     return oracle.err_metrics(C, R, S)
+
+
+def sample_rows(W, rows, rp=None, vals=None):
+    """Rows `rows` of a bitmap as their own bitmap: (W_sub, rp_sub, vals_sub)
+    (rp_sub / vals_sub None for a 0/1 mask)."""
+    Ws = np.ascontiguousarray(W[rows])
+    if rp is None:
+        return Ws, None, None
+    cnt = rp[rows + 1] - rp[rows]
+    rps = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    vs = (np.concatenate([vals[rp[r]:rp[r + 1]] for r in rows]).astype(np.float32)
+          if vals is not None else None)
+    return Ws, rps, vs
+
+
+def sampled_err(oracle, C_dev, W, K, B, rows, rp=None, vals=None):
+    """Error metrics of rows `rows` of the device result C_dev against the
+    FP64 oracle (B: the exact host B the reference uses)."""
+    Ws, rps, vs = sample_rows(W, rows, rp, vals)
+    Cs = C_dev[torch.from_numpy(rows).to(C_dev.device)].cpu().numpy()
+    return rel_err(oracle, Cs, Ws, K, B, rps, vs)

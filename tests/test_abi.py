@@ -147,8 +147,10 @@ def test_sddmm_and_groups_validation(bws):
 def test_workspace_sizes(bws):
     """Workspace queries: on a 0/1 A with N % 8 == 0 and a plan that allows
     MFMA, BF16X2 asks for B's two BF16 planes and BF16 / FP16 for one rounded
-    copy of B (used when B is FP32), nothing otherwise; the SDDMM workspace
-    holds the per-word offsets and one byte per 64 x 64 tile."""
+    copy of B (used when B is FP32); FP32 with N % 4 == 0 asks for A's mask
+    stream (8 bytes per row and 64-row stage, rows padded to 256); nothing
+    otherwise; the SDDMM workspace holds the per-word offsets and one byte per
+    64 x 64 tile."""
     L = bws.lib()
     P = bws.PRECISION
     a = _bm(300, 200, 7)
@@ -157,7 +159,10 @@ def test_workspace_sizes(bws):
     assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16"], None) >= 200 * 64 * 2
     assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp16"], None) >= 200 * 64 * 2
     assert L.bws_spmm_workspace_size(C.byref(a), 60, P["bf16"], None) == 0
-    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp32"], None) == 0
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp32"], None) == 512 * 4 * 8
+    assert L.bws_spmm_workspace_size(C.byref(a), 62, P["fp32"], None) == 0
+    assert L.bws_spmm_workspace_size(C.byref(a), 64, P["fp32"],
+                                     C.byref(bws.make_plan(kernel=1, rows_per_wg=128))) == 0
     plan = bws.make_plan(kernel=1)
     assert L.bws_spmm_workspace_size(C.byref(a), 64, P["bf16x2"], C.byref(plan)) == 0
     av = _bm(300, 200, 7, row_ptr=FAKE, values=FAKE)
@@ -183,8 +188,8 @@ def test_transpose_perm_gather_validation(bws):
 def test_cluster_validation(bws):
     L = bws.lib()
     a = _bm(64, 64, 2)
-    ws = L.bws_cluster_workspace_size(64, 0)
-    assert ws > 5 * 64 * 4 and L.bws_cluster_workspace_size(64, -1) == 0
+    ws = L.bws_cluster_workspace_size(64, 64, 0)
+    assert ws > 5 * 64 * 4 and L.bws_cluster_workspace_size(64, 64, -1) == 0
     assert L.bws_cluster_rows(C.byref(_bm(64, 64, 2, row_ptr=FAKE, values=FAKE)), 0, FAKE, FAKE,
                               FAKE, ws, None) == bws.BWS_EUNSUPPORTED
     assert L.bws_cluster_rows(C.byref(a), -1, FAKE, FAKE, FAKE, ws, None) == bws.BWS_EINVAL

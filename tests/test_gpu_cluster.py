@@ -15,7 +15,11 @@ pytestmark = pytest.mark.gpu
 
 PLAN_CASES = [(1, 1, 0, False, 0.5), (15, 33, 0, False, 0.3), (17, 100, 3, True, 0.1),
               (256, 64, 0, True, 0.2), (300, 1000, 7, False, 0.05), (1000, 513, 0, True, 0.05),
-              (2048, 2048, 0, True, 0.03125), (512, 768, 512, False, 0.05), (129, 40, 1, True, 0.3)]
+              (2048, 2048, 0, True, 0.03125), (512, 768, 512, False, 0.05), (129, 40, 1, True, 0.3),
+              # five 1024-row ranking blocks, more than 256 leaders (stride leaders)
+              (5000, 300, 300, True, 0.05), (3000, 96, 257, False, 0.1),
+              # the benchmark mask: 64 leaders searched over 256 candidates
+              (4096, 4096, 0, True, 0.03125)]
 
 
 def _bitmap(bws, dev, M, K, p, clustered, seed):
@@ -34,6 +38,32 @@ def test_plan_matches_oracle(oracle, bws, dev, M, K, L, clustered, p):
     # same plan twice (deterministic)
     pl2 = bws.cluster_rows(A, max_clusters=L)
     assert torch.equal(pl.slot_row, pl2.slot_row) and torch.equal(pl.chunk_mask, pl2.chunk_mask)
+
+
+def test_plan_graph_capture(oracle, bws, dev):
+    """The whole plan captured into one hipGraph (no host synchronisation, no
+    memset node): three replays, with eager work between them, each equal to
+    the oracle's plan."""
+    M, K = 3000, 1000
+    W, A = _bitmap(bws, dev, M, K, 0.05, True, seed=11)
+    slot, cm = oracle.cluster_rows(W, K)
+    bws.cluster_rows(A)  # warm-up outside the capture
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            pl = bws.cluster_rows(A)
+    torch.cuda.current_stream().wait_stream(s)
+    for rep in range(3):
+        pl.slot_row.fill_(-1)
+        pl.chunk_mask.fill_(0)
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(pl.slot_row.cpu().numpy(), slot), rep
+        assert np.array_equal(to_host_u32(pl.chunk_mask), cm), rep
+        torch.randn(1 << 20, device=dev).cumsum(0)  # eager work between replays
+        torch.cuda.synchronize()
 
 
 def test_plan_padded_ldw(oracle, bws, dev):

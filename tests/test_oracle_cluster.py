@@ -8,8 +8,10 @@ the FP32 product), so it is pinned by what fixes it:
    chunks of identical rows, base 0 equal to the pattern, no base 1, and
    every row a zero-difference delta row;
 3. outputs checked from first principles on random masks: slot_row is a
-   permutation in farthest-point leader order, each base is the majority of
-   its run, each flag follows the popcount rule;
+   permutation in nearest-leader order (leaders by farthest point over the
+   stride-sampled candidate rows, or the candidates themselves past 256
+   leaders), each base is the majority of its run, each flag follows the
+   popcount rule; the 65535 clamp of the search distance decides a tie;
 4. the algebra the clustered product relies on: T + (row & ~base)·B -
    (base & ~row)·B equals row·B, evaluated in FP64 against the FP64 oracle."""
 import numpy as np
@@ -105,8 +107,46 @@ def test_exact_clusters(oracle, rng, patterns, reps, cols):
         assert (cm[c, 1] == 0).all()
 
 
+def rederive_owners(b, nl):
+    """Leaders and nearest-leader owners restated from the rules: the last
+    R = min(8, L - 1) leaders by exact farthest point over all rows (lowest
+    row on ties); the first Lc = L - R by farthest point over the candidates
+    j * rows // c (distance clamped at 65535, lowest candidate on ties) when
+    Lc <= 256, else the candidates themselves."""
+    import oracle
+    rows = b.shape[0]
+    R = min(8, nl - 1)
+    assert R == oracle.cluster_exact_rounds(nl)
+    Lc = nl - R
+    c = Lc if Lc > 256 else min(rows, 256)
+    assert c == oracle.cluster_candidates(rows, nl)
+    cand = [j * rows // c for j in range(c)]
+    d = lambda r, s: int((b[r] ^ b[s]).sum())
+    if Lc > 256:
+        leads = cand[:Lc]
+    else:
+        leads = [cand[0]]
+        near = [min(d(x, cand[0]), 65535) for x in cand]
+        for _ in range(1, Lc):
+            j = int(np.argmax(near))  # first maximum: the lowest candidate
+            leads.append(cand[j])
+            near = [min(near[i], min(d(x, cand[j]), 65535)) for i, x in enumerate(cand)]
+    dist = [min(d(r, l) for l in leads) for r in range(rows)]
+    for _ in range(R):
+        r = int(np.argmax(dist))  # first maximum: the lowest row
+        leads.append(r)
+        dist = [min(dist[x], d(x, r)) for x in range(rows)]
+    owner = []
+    for r in range(rows):
+        ds = [d(r, l) for l in leads]
+        owner.append(int(np.argmin(ds)))  # first minimum: the earliest leader
+    return owner
+
+
 @pytest.mark.parametrize("rows,cols,L,p", [(37, 45, 0, 0.2), (100, 64, 5, 0.1), (333, 129, 0, 0.05),
-                                           (16, 300, 16, 0.3), (1, 10, 0, 0.5), (50, 33, 50, 0.4)])
+                                           (16, 300, 16, 0.3), (1, 10, 0, 0.5), (50, 33, 50, 0.4),
+                                           (700, 40, 9, 0.2), (600, 70, 300, 0.1),
+                                           (900, 50, 20, 0.15)])
 def test_first_principles(oracle, rng, rows, cols, L, p):
     b = rng.random((rows, cols)) < p
     b[rng.random(rows) < 0.1] = False  # some empty rows
@@ -115,16 +155,7 @@ def test_first_principles(oracle, rng, rows, cols, L, p):
     order = slot & ROW
     assert sorted(order.tolist()) == list(range(rows))
     nl = oracle.cluster_leaders(rows, L)
-    # farthest-point leaders and nearest-leader owners, restated from the rules
-    d = lambda r, s: int((b[r] ^ b[s]).sum())
-    dist = [d(r, 0) for r in range(rows)]
-    owner = [0] * rows
-    lead = int(np.argmax(dist))
-    for i in range(1, nl):
-        for r in range(rows):
-            if d(r, lead) < dist[r]:
-                dist[r], owner[r] = d(r, lead), i
-        lead = int(np.argmax(dist))
+    owner = rederive_owners(b, nl)
     expect = sorted(range(rows), key=lambda r: (owner[r], r))
     assert order.tolist() == expect
     valid = words(np.ones((1, cols), bool))[0]
@@ -176,6 +207,35 @@ def test_delta_algebra_fp64(oracle, bws):
         else:
             Cp[r] = oracle.spmm_f64(W[r:r + 1], K, B)[0]
     assert np.abs(Cp - R).max() <= 1e-12 * max(1.0, np.abs(R).max())
+
+
+def test_search_distance_clamp(oracle):
+    """Distances above 65535 tie in the candidate search.  Row 0 empty, row 1
+    at distance 69990 and row 2 at 70000 from it, row 3 five bits, rows 4..11
+    disjoint blocks of 1000 + i bits.  With 11 leaders (3 searched, 8 exact
+    rounds) the search takes rows 0, 1 (the lower of the tied candidates) and
+    11; the exact rounds then rows 10 .. 4 and last row 2.  Unclamped, row 2
+    would lead second and row 1 last, swapping their slots."""
+    cols = 70000
+    b = np.zeros((12, cols), bool)
+    b[1, 10:] = True
+    b[2, :] = True
+    b[3, :5] = True
+    for i in range(8):
+        b[4 + i, 10000 + 1100 * i:11000 + 1101 * i] = True
+    assert [int(b[4 + i].sum()) for i in range(8)] == [1000 + i for i in range(8)]
+    W = words(b)
+    slot, cm = oracle.cluster_rows(W, cols, max_clusters=11)
+    assert (slot & ROW).tolist() == [0, 3, 1, 11, 10, 9, 8, 7, 6, 5, 4, 2]
+    assert rederive_owners(b, 11) == [0, 1, 10, 0, 9, 8, 7, 6, 5, 4, 3, 2]
+
+
+def test_candidate_count(oracle):
+    assert oracle.cluster_candidates(0, 0) == 0
+    assert oracle.cluster_candidates(4096, 64) == 256
+    assert oracle.cluster_candidates(100, 1) == 100
+    assert oracle.cluster_candidates(100000, 1000) == 992
+    assert [oracle.cluster_exact_rounds(L) for L in (0, 1, 2, 9, 10, 64)] == [0, 0, 1, 8, 8, 8]
 
 
 def test_leader_count(oracle):

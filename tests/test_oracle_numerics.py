@@ -107,6 +107,32 @@ def test_symbolic_or_vs_boolean_product(oracle, rng):
     assert np.array_equal(WC, oracle.pack_bitmap(expect.astype(np.float32)))
 
 
+@pytest.mark.parametrize("a_vals,b_vals", [(True, True), (False, True), (True, False),
+                                           (False, False)])
+def test_spgemm_f64_vs_scipy_sparse(oracle, rng, a_vals, b_vals):
+    """The numeric sparse x sparse product onto the symbolic pattern equals
+    scipy's CSR product read at the pattern's set bits (row-major order); a
+    pattern superset reads explicit zeros there."""
+    import scipy.sparse as sp
+    M, K, N = 37, 70, 90
+    DA = rand_mask_values(rng, M, K, 0.1, a_vals)
+    DB = rand_mask_values(rng, K, N, 0.05, b_vals)
+    WA, WB = oracle.pack_bitmap(DA), oracle.pack_bitmap(DB)
+    rpA, rpB = oracle.row_ptr(WA, K), oracle.row_ptr(WB, N)
+    vA = oracle.pack_values(DA, WA, rpA) if a_vals else None
+    vB = oracle.pack_values(DB, WB, rpB) if b_vals else None
+    ref = (sp.csr_matrix(DA.astype(np.float64)) @ sp.csr_matrix(DB.astype(np.float64))).toarray()
+    for WC in (oracle.symbolic_or(WA, K, WB, N),
+               oracle.pack_bitmap((rng.random((M, N)) < 0.5).astype(np.float32))):
+        rpC = oracle.row_ptr(WC, N)
+        got = oracle.spgemm_f64(WA, K, WB, N, WC, rpC, rpA, vA, rpB, vB)
+        mask = oracle.unpack_bitmap(WC, N) != 0
+        np.testing.assert_allclose(got, ref[mask], rtol=1e-13, atol=1e-13)
+        S = oracle.spgemm_f64(WA, K, WB, N, WC, rpC, rpA, vA, rpB, vB, absolute=True)
+        refS = (abs(sp.csr_matrix(DA.astype(np.float64))) @ abs(sp.csr_matrix(DB.astype(np.float64)))).toarray()
+        np.testing.assert_allclose(S, refS[mask], rtol=1e-13, atol=1e-13)
+
+
 def test_err_metrics_closed_form(oracle):
     R = np.array([[1.0, -2.0, 0.0, 1e-9]])
     S = np.array([[2.0, 4.0, 0.0, 1.0]])
