@@ -138,6 +138,8 @@ _SIGS = {
     "bws_spmm_workspace_size": ([_BP, _I64, _I32, C.POINTER(_Plan)], _SZ),
     "bws_spmm": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P, _SZ,
                   _P], _I32),
+    "bws_spmm_fp32_noalloc_max_bytes": ([], _SZ),
+    "bws_spmm_fp32_noalloc": ([_BP, _P, _I64, _I64, _P, _I64, _F, _F, _P], _I32),
     "bws_spmm_groups": ([_BP, _P, _P, _I32, _I32, _P, _I64, _I64, _P, _I64, _P], _I32),
     "bws_spmm_tA": ([_BP, _P, _I32, _I64, _I64, _P, _I64, _F, _F, _I32, C.POINTER(_Plan), _P,
                      _SZ, _P], _I32),
@@ -689,6 +691,26 @@ def spmm(A: Bitmap, B: torch.Tensor, precision: str = "fp32", alpha: float = 1.0
                           out.stride(0), alpha, beta, PRECISION[precision], pl, _ptr(ws),
                           0 if ws is None else ws.numel(), _stream(dev)),
            "bws_spmm")
+    return out
+
+
+def spmm_fp32_noalloc(A: Bitmap, B: torch.Tensor, alpha: float = 1.0, beta: float = 0.0,
+                      out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """C = alpha * A·B + beta * C at FP32 through bws_spmm_fp32_noalloc: the
+    library allocates nothing (the mask stream lives in its own device array;
+    calls overlapping on one device race on it)."""
+    _require_gpu(B)
+    assert B.dtype == torch.float32 and B.dim() == 2 and B.shape[0] == A.cols
+    assert B.stride(1) == 1 or B.shape[1] <= 1
+    N = B.shape[1]
+    if out is None:
+        out = torch.empty((A.rows, N), dtype=torch.float32, device=B.device)
+        if beta != 0.0:
+            raise ValueError("beta != 0 needs `out`")
+    s = A.struct()
+    _check(lib().bws_spmm_fp32_noalloc(C.byref(s), _ptr(B), B.stride(0), N, _ptr(out),
+                                       out.stride(0), alpha, beta, _stream(B.device)),
+           "bws_spmm_fp32_noalloc")
     return out
 
 

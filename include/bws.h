@@ -377,6 +377,23 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
                     int32_t precision, const bws_plan* plan, void* workspace,
                     size_t workspace_bytes, bws_stream stream);
 
+/* bws_spmm at BWS_PREC_FP32 with an F32 B and a 0/1 A, allocating nothing and
+ * taking no workspace: the mask stream the FP32 workspace would hold lives
+ * in a device array of the library, bws_spmm_fp32_noalloc_max_bytes() bytes
+ * (16 MiB) on each device it runs on, held until the process ends.  A whose
+ * mask stream exceeds it (bws_spmm_workspace_size(A, N, BWS_PREC_FP32, NULL)
+ * above the maximum; about rows * cols / 8 bytes, e.g. 8192 x 16384) fails
+ * with BWS_ECAPACITY.  Same results as bws_spmm with that workspace.  Every
+ * call on a device writes the array before reading it, and calls that
+ * overlap on one device (several streams or host threads) race on it.
+ * No counterpart in the reference: it exists for callers that cannot
+ * allocate (code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc). */
+size_t bws_spmm_fp32_noalloc_max_bytes(void);
+bws_status bws_spmm_fp32_noalloc(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N,
+                                 float* C, int64_t ldc, float alpha, float beta,
+                                 bws_stream stream);
+
 /* Grouped evaluation: C = A·B where each nonzero's value is read from the
  * group tables (group_vals of its row's group) and spilled rows' values from
  * A.values (or 1.0).  Equals bws_spmm on the same A (FP32 path), since group
