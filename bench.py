@@ -217,11 +217,16 @@ def main():
     if path == "fp32_rowclusters":  # + slot rows and chunk bases
         alg_bytes += 4 * M + 4 * 2 * ((M + 15) // 16) * ((K + 31) // 32)
     achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
+    kname = {"fp32_rowclusters": "k_spmm_stream<64, 2, true>",
+             "fp32": "k_spmm_stream<64, 2, false>"}.get(path)
     traffic = None
     tfile = PROFILE_TRAFFIC.format(path)
     if os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125:
         try:
-            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+            tj = json.load(open(tfile))
+            # only a counter reading of the kernel that runs now
+            if kname is None or tj.get("kernel") == kname:
+                traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     t_hbm_us = alg_bytes / (HBM_PEAK_GBS * 1e9) * 1e6
@@ -229,7 +234,7 @@ def main():
         # binding ceiling of the vector-add path: max(HBM time, nnz*N adds)
         t_add_us = nnz * N / ADD_RATE_PACKED * 1e6
         t_add_unpacked_us = nnz * N / ADD_RATE_UNPACKED * 1e6
-        roofline = {"kernel": "k_spmm_cluster" if path == "fp32_rowclusters" else "k_spmm_rows2",
+        roofline = {"kernel": kname,
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel_ms": round(ms_kernel, 5), "algorithmic_bytes": alg_bytes,
