@@ -15,20 +15,24 @@
 // per chunk and 64-row stage in the layout the streamed kernel reads.
 //
 // Plan (device only, no host synchronisation, no memset: capturable):
-//  1. leaders: c stride-sampled candidate rows (c = 256 for up to 256 leaders);
-//     their pairwise Hamming distances (clamped to 65535) in a c x c table,
-//     then one workgroup runs the farthest-point search over the table in
-//     LDS.  More than 256 leaders: the candidates themselves;
-//  2. every row's nearest leader: a wave per row, a lane per leader;
+//  1. leaders: c stride-sampled candidate rows (c = 256 for up to 256 leaders
+//     from the search); their pairwise Hamming distances (clamped to 65535)
+//     in a c x c table, a block per candidate; then one wave runs the
+//     farthest-point search over the table in LDS.  More leaders than 256:
+//     the candidates themselves;
+//  2. every row's nearest of those leaders (16 rows per block pass, the
+//     leaders' words staged in LDS, the rows' read with scalar loads), then
+//     R = min(8, L - 1) exact farthest-point rounds over all rows: each round
+//     is one launch that reduces the previous pass's per-block keys to its
+//     leader (no same-address atomics);
 //  3. rows stably ordered by nearest leader: per 1024-row block a bitonic
 //     sort in LDS gives each row its rank among the block's rows of the same
-//     leader and the block's count per leader; one exclusive scan over
-//     (leader, block) gives every row its slot;
-//  4. chunk runs: a thread per chunk finds its two longest runs of equal
-//     leader; chunk masks: a thread per (chunk, base, word) counts the run's
-//     bits;
-//  5. slot flags: a wave per slot compares popcount(row ^ base_j) with
-//     popcount(row).
+//     leader and the block's count of every leader; an exclusive scan over
+//     (leader, block) gives every row its slot (done by each block of the
+//     placing kernel itself up to 8192 pairs);
+//  4. a block per chunk: its two longest runs of equal leader, each run's
+//     majority mask (bit-sliced counters) as the chunk's bases, then a wave
+//     per slot compares popcount(row ^ base_j) with popcount(row).
 #include <algorithm>
 
 #include "mfma_common.hpp"
@@ -68,73 +72,159 @@ int64_t candidates_for(int64_t rows, int32_t L) {
   return std::min<int64_t>(rows, kSearchMax);
 }
 
-__device__ __forceinline__ int64_t cand_row(int64_t j, int64_t rows, int64_t c) {
+__host__ __device__ __forceinline__ int64_t cand_row(int64_t j, int64_t rows, int64_t c) {
   return j * rows / c;
 }
 
-// Transposed words of candidate (or leader) rows: T[w * n + j] = word w of
-// row rowsel(j), bits past K cleared.  A thread per (w, j).
+// Rows per block pass of the nearest-leader kernels (4 per wave), and the
+// cap on their grid (the exact rounds reduce one key per block).
+constexpr int kPassRows = 16;
+constexpr int64_t kMaxPassBlocks = 1024;
+int64_t pass_blocks(int64_t rows) { return std::min<int64_t>(ceil_div(rows, kPassRows), kMaxPassBlocks); }
+
+typedef const __attribute__((address_space(4))) uint32_t* sgpr_src;
+
+// ---------------------------------------------------------------------------
+// wave and block reductions
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+struct OpMax {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
+struct OpMin {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a < b ? a : b; }
+};
+struct OpAdd {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+// 64-lane reduction, result uniform: xor 1 and xor 2 (quad permutes), then the
+// half-row and row mirrors leave every lane of a 16-lane row with the row's
+// result; the four rows are combined from lanes 0, 16, 32 and 48.  Every lane
+// of the wave must be active.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v, Op op) {
+  v = op(v, dpp<0xB1>(v));
+  v = op(v, dpp<0x4E>(v));
+  v = op(v, dpp<0x141>(v));
+  v = op(v, dpp<0x140>(v));
+  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+  const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+  const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+  const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+  return op(op(a, b), op(c, d));
+}
+__device__ __forceinline__ uint64_t max_u64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  return ((uint64_t)dpp<CTRL>(hi32(v)) << 32) | dpp<CTRL>(lo32(v));
+}
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int l) {
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi32(v), l) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)lo32(v), l);
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  v = max_u64(v, dpp64<0xB1>(v));
+  v = max_u64(v, dpp64<0x4E>(v));
+  v = max_u64(v, dpp64<0x141>(v));
+  v = max_u64(v, dpp64<0x140>(v));
+  return max_u64(max_u64(lane_u64(v, 0), lane_u64(v, 16)), max_u64(lane_u64(v, 32), lane_u64(v, 48)));
+}
+// Max over a 256-thread block (every thread reaches it), returned to all.
+__device__ __forceinline__ uint64_t block_max_u64(uint64_t v, uint64_t* sh) {
+  v = wave_max_u64(v);
+  __syncthreads();  // sh may still be read from an earlier call
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return max_u64(max_u64(sh[0], sh[1]), max_u64(sh[2], sh[3]));
+}
+
+// ---------------------------------------------------------------------------
+// 1. leaders
+// ---------------------------------------------------------------------------
+constexpr int kDistWords = 128;  // words of a candidate row per pass
+
+// D[i * c + j] = min(Hamming(candidate i, candidate j), 65535): a block per
+// candidate i, its words staged in LDS 128 at a time; a thread per candidate
+// j loads its own 128 words at once (32 16-byte loads when VEC: ldw % 4 == 0
+// and W 16-byte aligned) and counts against the staged words.
+template <bool VEC>
 __global__ void __launch_bounds__(256)
-k_rows_t(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-         const int32_t* __restrict__ rowsel, int64_t n, uint32_t* __restrict__ T) {
+k_cand_dist(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw, int64_t c,
+            uint16_t* __restrict__ D) {
+  __shared__ __align__(16) uint32_t si[kDistWords];
+  const int64_t i = blockIdx.x;
+  const int j = threadIdx.x;
   const int64_t wpr = words_for(K);
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= wpr * n) return;
-  const int64_t w = t / n, j = t - w * n;
-  const int64_t r = rowsel ? (int64_t)rowsel[j] : cand_row(j, M, n);
-  T[t] = W[r * ldw + w] & valid_bits(w, K);
+  const uint32_t* ri = W + cand_row(i, M, c) * ldw;
+  const uint32_t* rj = W + cand_row(j < c ? j : 0, M, c) * ldw;
+  uint32_t d = 0;
+  for (int64_t w0 = 0; w0 < wpr; w0 += kDistWords) {
+    const int n = (int)std::min<int64_t>(kDistWords, wpr - w0);
+    __syncthreads();
+    if (j < kDistWords) si[j] = j < n ? ri[w0 + j] & valid_bits(w0 + j, K) : 0u;
+    __syncthreads();
+    if (j < c) {
+      if (VEC && n == kDistWords) {
+        uint4 x[kDistWords / 4];
+#pragma unroll
+        for (int s = 0; s < kDistWords / 4; ++s) x[s] = reinterpret_cast<const uint4*>(rj + w0)[s];
+#pragma unroll
+        for (int s = 0; s < kDistWords / 4; ++s) {
+          const uint4 y = reinterpret_cast<const uint4*>(si)[s];
+          d += __popc(y.x ^ x[s].x) + __popc(y.y ^ x[s].y) + __popc(y.z ^ x[s].z) +
+               __popc(y.w ^ x[s].w);
+        }
+      } else {
+#pragma unroll 8
+        for (int e = 0; e < n; ++e) d += __popc(si[e] ^ rj[w0 + e]);
+      }
+    }
+  }
+  if (j < c) {
+    if (wpr > 0) d -= __popc(rj[wpr - 1] & ~valid_bits(wpr - 1, K));  // row j's bits past K
+    D[i * c + j] = (uint16_t)(d < 65535u ? d : 65535u);
+  }
 }
 
-// D[i * c + j] = min(Hamming(candidate i, candidate j), 65535): a wave per
-// (i, 64 candidates j), a lane per j.
-__global__ void __launch_bounds__(256)
-k_cand_dist(const uint32_t* __restrict__ T, int64_t wpr, int64_t c, uint16_t* __restrict__ D) {
-  const int lane = threadIdx.x & 63;
-  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int64_t jb = ceil_div(c, 64);
-  if (g >= c * jb) return;
-  const int64_t i = g / jb, j = (g - i * jb) * 64 + lane;
-  if (j >= c) return;
-  int64_t d = 0;
-  for (int64_t w = 0; w < wpr; ++w) d += __popc(T[w * c + i] ^ T[w * c + j]);
-  D[i * c + j] = (uint16_t)(d < 65535 ? d : 65535);
-}
-
-// Farthest-point search over c <= 256 candidates, one workgroup of 256
-// threads (thread j = candidate j), the distance table in LDS: leader 0 is
-// candidate 0; each candidate keeps its distance to the nearest leader so
-// far; the next leader is the candidate with the largest (lowest j on ties).
+// Farthest-point search over c <= 256 candidates: the block stages the table
+// in LDS, then one wave runs the search (lane l holds candidates l, l + 64,
+// l + 128, l + 192): leader 0 is candidate 0; each candidate keeps its
+// distance to the nearest leader so far; the next leader is the candidate
+// with the largest (lowest index on ties).  Leaders' rows are written after
+// the search.
 __global__ void __launch_bounds__(kSearchMax)
 k_leader_search(const uint16_t* __restrict__ D, int64_t c, int64_t M, int32_t L,
                 int32_t* __restrict__ lead_row) {
   __shared__ uint16_t sD[kSearchMax * kSearchMax];
-  __shared__ uint32_t best[kSearchMax / 64];
-  const int j = threadIdx.x, lane = j & 63, wave = j >> 6;
-  for (int64_t e = j; e < c * c; e += kSearchMax) sD[e] = D[e];
+  __shared__ int32_t sidx[kSearchMax];
+  const int64_t n = c * c, n8 = n / 8;
+  for (int64_t e = threadIdx.x; e < n8; e += kSearchMax)  // D is 256-byte aligned
+    reinterpret_cast<uint4*>(sD)[e] = reinterpret_cast<const uint4*>(D)[e];
+  for (int64_t e = n8 * 8 + threadIdx.x; e < n; e += kSearchMax) sD[e] = D[e];
   __syncthreads();
-  uint32_t near = 0;
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const int cc = (int)c;
+  uint32_t near[4] = {0u, 0u, 0u, 0u};
   int lead = 0;
   for (int32_t it = 0; it < L; ++it) {
-    if (j == 0) lead_row[it] = (int32_t)cand_row(lead, M, c);
-    if (j < c) {
-      const uint32_t d = sD[lead * c + j];
-      near = (it == 0 || d < near) ? d : near;
-    }
-    // argmax of near, lowest candidate on ties: max of (near << 16 | ~j)
-    uint32_t key = j < c ? (near << 16) | (uint32_t)(0xFFFF - j) : 0u;
+    if (lane == 0) sidx[it] = lead;
+    const uint16_t* row = sD + lead * cc;
+    uint32_t key = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint32_t v = (uint32_t)__shfl_xor((int)key, o, 64);
-      key = v > key ? v : key;
+    for (int q = 0; q < 4; ++q) {
+      const int j = lane + 64 * q;
+      const uint32_t d = j < cc ? row[j] : 0u;
+      near[q] = (it == 0 || d < near[q]) ? d : near[q];
+      const uint32_t k = j < cc ? (near[q] << 16) | (uint32_t)(0xFFFF - j) : 0u;
+      key = k > key ? k : key;
     }
-    if (lane == 0) best[wave] = key;
-    __syncthreads();
-    uint32_t m = best[0];
-#pragma unroll
-    for (int q = 1; q < kSearchMax / 64; ++q) m = best[q] > m ? best[q] : m;
-    lead = 0xFFFF - (int)(m & 0xFFFF);
-    __syncthreads();  // best[] is rewritten next iteration
+    lead = 0xFFFF - (int)(wave_reduce(key, OpMax()) & 0xFFFF);
   }
+  for (int it = lane; it < L; it += 64) lead_row[it] = (int32_t)cand_row(sidx[it], M, c);
 }
 
 // More leaders than the search takes: leader i is candidate i.
@@ -144,106 +234,188 @@ k_stride_leaders(int64_t M, int32_t L, int32_t* __restrict__ lead_row) {
   if (i < L) lead_row[i] = (int32_t)cand_row(i, M, L);
 }
 
-// Block max of a 64-bit key folded into *slot (one atomicMax per block).
-__device__ __forceinline__ void block_max_to(unsigned long long key, unsigned long long* slot) {
-  __shared__ unsigned long long best[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long v = (unsigned long long)__shfl_xor((long long)key, o, 64);
-    key = v > key ? v : key;
-  }
-  if (lane == 0) best[wave] = key;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = best[0];
-    for (int i = 1; i < 4; ++i) m = best[i] > m ? best[i] : m;
-    if (m) atomicMax(slot, m);
-  }
-}
+// ---------------------------------------------------------------------------
+// 2. nearest leaders
+// ---------------------------------------------------------------------------
+constexpr int kLeadWords = 128;  // words per LDS pass
+constexpr int kLeadPitch = 65;   // padded LDS row (one word of 64 leaders)
 
 // Nearest of the first Lc leaders for every row (full Hamming distance,
-// earliest leader on ties): a wave per row, a lane per leader of each group of
-// 64; LT holds the leaders' words transposed (LT[w * Lc + i]).  The row's
-// distance goes to dist, and the block's max of (distance, ~row) to *far —
-// the first exact round's leader.
+// earliest leader on ties).  A block takes 16 rows at a time, 4 per wave;
+// lane i of a wave is leader g + i of a group of 64.  Per pass of 128 words
+// the block stages the group's leader words (transposed) and its rows' words
+// in LDS, bits past K cleared.  The row's distance goes to dist and the
+// block's max of (distance, ~row) over its rows to part[block] (reduced by
+// the first exact round).
 __global__ void __launch_bounds__(256)
 k_assign(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-         const uint32_t* __restrict__ LT, int32_t Lc, int32_t* __restrict__ owner,
-         int32_t* __restrict__ dist, unsigned long long* __restrict__ far) {
-  const int lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+         const int32_t* __restrict__ lead_row, int32_t Lc, int32_t* __restrict__ owner,
+         int32_t* __restrict__ dist, unsigned long long* __restrict__ part) {
+  __shared__ uint32_t sl[kLeadWords * kLeadPitch];
+  __shared__ __align__(16) uint32_t srw[kPassRows][kLeadWords];
+  __shared__ int32_t slr[64];
+  __shared__ uint64_t sh[4];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t wpr = words_for(K);
-  unsigned long long far_key = 0ull;
-  if (r < M) {
-    uint64_t bestkey = ~0ull;
+  uint64_t far_key = 0ull;
+  for (int64_t rb = blockIdx.x; rb * kPassRows < M; rb += gridDim.x) {
+    uint32_t best_d[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t best_i[4] = {0u, 0u, 0u, 0u};
     for (int32_t g = 0; g < Lc; g += 64) {
-      const int32_t i = g + lane;
-      int64_t d = 0;
-      if (i < Lc)
-        for (int64_t w = 0; w < wpr; ++w)
-          d += __popc((W[r * ldw + w] & valid_bits(w, K)) ^ LT[w * Lc + i]);
-      uint64_t key = i < Lc ? ((uint64_t)d << 32) | (uint32_t)i : ~0ull;
+      __syncthreads();  // slr is rewritten
+      if (t < 64) slr[t] = lead_row[g + t < Lc ? g + t : g];
+      uint32_t d[4] = {0u, 0u, 0u, 0u};
+      for (int64_t w0 = 0; w0 < wpr; w0 += kLeadWords) {
+        const int n = (int)std::min<int64_t>(kLeadWords, wpr - w0);
+        __syncthreads();  // slr written; the previous pass's readers are done
+        {
+          const int k = t & 127, i0 = t >> 7;
+          const uint32_t vb = k < n ? valid_bits(w0 + k, K) : 0u;
+          uint32_t v[32];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const uint64_t v = (uint64_t)__shfl_xor((long long)key, o, 64);
-        key = v < key ? v : key;
+          for (int s = 0; s < 32; ++s)
+            v[s] = k < n ? W[(int64_t)slr[i0 + 2 * s] * ldw + w0 + k] & vb : 0u;
+#pragma unroll
+          for (int s = 0; s < 32; ++s) sl[k * kLeadPitch + i0 + 2 * s] = v[s];
+          uint32_t x[8];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const int row = (t >> 7) + 2 * s;
+            const int64_t r = rb * kPassRows + row;
+            x[s] = (k < n && r < M) ? W[r * ldw + w0 + k] & vb : 0u;
+          }
+#pragma unroll
+          for (int s = 0; s < 8; ++s) srw[(t >> 7) + 2 * s][k] = x[s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t* rw = srw[wave * 4 + q];
+          uint32_t acc = d[q];
+          for (int k = 0; k < n; k += 4) {
+            const uint4 y = *reinterpret_cast<const uint4*>(rw + k);
+            acc += __popc(sl[k * kLeadPitch + lane] ^ y.x) +
+                   __popc(sl[(k + 1) * kLeadPitch + lane] ^ y.y) +
+                   __popc(sl[(k + 2) * kLeadPitch + lane] ^ y.z) +
+                   __popc(sl[(k + 3) * kLeadPitch + lane] ^ y.w);
+          }
+          d[q] = acc;
+        }
       }
-      bestkey = key < bestkey ? key : bestkey;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t v = g + lane < Lc ? d[q] : 0xFFFFFFFFu;
+        const uint32_t m = wave_reduce(v, OpMin());
+        const uint32_t f = (uint32_t)__builtin_ctzll(__ballot(v == m));
+        if (m < best_d[q]) {
+          best_d[q] = m;
+          best_i[q] = (uint32_t)g + f;
+        }
+      }
     }
-    const uint32_t d = (uint32_t)(bestkey >> 32);
-    if (lane == 0) {
-      owner[r] = (int32_t)(uint32_t)bestkey;
-      dist[r] = (int32_t)d;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t r = rb * kPassRows + wave * 4 + q;
+      if (r < M) {
+        if (lane == 0) {
+          owner[r] = (int32_t)best_i[q];
+          dist[r] = (int32_t)best_d[q];
+        }
+        far_key = max_u64(far_key, ((uint64_t)best_d[q] << 32) | (uint32_t)~(uint32_t)r);
+      }
     }
-    far_key = ((unsigned long long)d << 32) | (uint32_t)~(uint32_t)r;
   }
-  block_max_to(far_key, far);
+  const uint64_t bm = block_max_u64(far_key, sh);
+  if (t == 0) part[blockIdx.x] = bm;
 }
 
-// Exact round `it`: leader Lc + it is the row recorded in far[it] (the row
-// farthest from its nearest leader, lowest row on ties); every row takes its
-// distance to it (strictly nearer: the earlier leader keeps ties) and the
-// block's max of (distance, ~row) goes to far[it + 1].
+// Exact round `it`: leader Lc + it is the row of the largest of the previous
+// pass's block keys prev[0..np) (the row farthest from its nearest leader,
+// lowest row on ties); every row takes its distance to it (strictly nearer:
+// the earlier leader keeps ties) and the block's max of (distance, ~row) over
+// its rows goes to next[block].  16 rows per block pass, 4 per wave, a lane
+// per word; the first pass's first 128 words of each row are loaded before
+// the leader is known.
 __global__ void __launch_bounds__(256)
 k_exact_round(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw, int32_t Lc,
-              int32_t it, unsigned long long* __restrict__ far, int32_t* __restrict__ owner,
+              int32_t it, const unsigned long long* __restrict__ prev, int64_t np,
+              unsigned long long* __restrict__ next, int32_t* __restrict__ owner,
               int32_t* __restrict__ dist) {
-  const int lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ uint64_t sh[4];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t wpr = words_for(K);
-  const int64_t lead = (int64_t)(uint32_t)~(uint32_t)far[it];
-  unsigned long long far_key = 0ull;
-  if (r < M) {
-    const int32_t prev = dist[r];
-    int32_t d = 0;
-    for (int64_t w = lane; w < wpr; w += 64)
-      d += __popc((W[r * ldw + w] ^ W[lead * ldw + w]) & valid_bits(w, K));
-    d = wave_sum(d);
-    const bool nearer = d < prev;
-    if (lane == 0 && nearer) {
-      dist[r] = d;
-      owner[r] = Lc + it;
+  const int64_t rb0 = blockIdx.x;
+  uint32_t pre[4][2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t r = std::min<int64_t>(rb0 * kPassRows + wave * 4 + q, M - 1);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int64_t w = lane + 64 * u;
+      pre[q][u] = w < wpr ? W[r * ldw + w] & valid_bits(w, K) : 0u;
     }
-    const uint32_t dd = (uint32_t)(nearer ? d : prev);
-    far_key = ((unsigned long long)dd << 32) | (uint32_t)~(uint32_t)r;
   }
-  block_max_to(far_key, far + it + 1);
+  uint64_t k0 = 0ull;
+  for (int64_t e = t; e < np; e += 256) k0 = max_u64(k0, prev[e]);
+  const int64_t lead = (int64_t)(uint32_t)~(uint32_t)block_max_u64(k0, sh);
+  const uint32_t* lw = W + lead * ldw;
+  uint64_t far_key = 0ull;
+  for (int64_t rb = rb0; rb * kPassRows < M; rb += gridDim.x) {
+    const int64_t r0 = rb * kPassRows + wave * 4;
+    uint32_t d[4] = {0u, 0u, 0u, 0u};
+    for (int64_t wb = 0; wb < wpr; wb += 128) {
+      const bool first = rb == rb0 && wb == 0;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int64_t w = wb + lane + 64 * u;
+        if (w < wpr) {
+          const uint32_t vb = valid_bits(w, K);
+          const uint32_t l = lw[w] & vb;
+          if (first) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d[q] += __popc(pre[q][u] ^ l);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int64_t r = r0 + q < M ? r0 + q : M - 1;
+              d[q] += __popc((W[r * ldw + w] & vb) ^ l);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t dq = wave_reduce(d[q], OpAdd());
+      const int64_t r = r0 + q;
+      if (r < M) {
+        const int32_t pd = dist[r];
+        const bool nearer = (int32_t)dq < pd;
+        if (lane == 0 && nearer) {
+          dist[r] = (int32_t)dq;
+          owner[r] = Lc + it;
+        }
+        const uint32_t dd = nearer ? dq : (uint32_t)pd;
+        far_key = max_u64(far_key, ((uint64_t)dd << 32) | (uint32_t)~(uint32_t)r);
+      }
+    }
+  }
+  const uint64_t bm = block_max_u64(far_key, sh);
+  if (t == 0) next[blockIdx.x] = bm;
 }
 
-__global__ void __launch_bounds__(256)
-k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
-    p[e] = v;
-}
-
+// ---------------------------------------------------------------------------
+// 3. stable order by leader
+// ---------------------------------------------------------------------------
 // Stable ranking of a 1024-row block by leader: bitonic sort of (leader << 10
 // | row in block) in LDS; a row's rank is its sorted position minus the start
-// of its leader's run, and each run's length is the block's count of that
-// leader: hist[leader * nb + block].  hist is zero elsewhere (filled before).
+// of its leader's run; the block's count of every leader l < L (zero when it
+// has none) goes to hist[l * nb + block], by binary search over the keys.
 __global__ void __launch_bounds__(kRankBlock)
-k_block_rank(const int32_t* __restrict__ owner, int64_t M, int64_t nb, int32_t* __restrict__ lrank,
-             int32_t* __restrict__ hist) {
+k_block_rank(const int32_t* __restrict__ owner, int64_t M, int64_t nb, int32_t L,
+             int32_t* __restrict__ lrank, int32_t* __restrict__ hist) {
   __shared__ uint32_t sk[kRankBlock];
   __shared__ int32_t ss[kRankBlock];
   const int t = threadIdx.x;
@@ -277,12 +449,21 @@ k_block_rank(const int32_t* __restrict__ owner, int64_t M, int64_t nb, int32_t* 
     ss[t] = v > ss[t] ? v : ss[t];
     __syncthreads();
   }
-  const uint32_t k_e = e >> 10;
   const int src = (int)(e & 1023u);
   const int64_t row = b * kRankBlock + src;
   if (row < M) lrank[row] = t - ss[t];
-  const bool tail = t == kRankBlock - 1 || (sk[t + 1] >> 10) != k_e;
-  if (tail && k_e != (uint32_t)kPadKey) hist[(int64_t)k_e * nb + b] = t - ss[t] + 1;
+  // first position whose leader is >= x
+  auto lower = [&](uint32_t x) {
+    int lo = 0, hi = kRankBlock;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((sk[mid] >> 10) < x) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  for (int64_t l = t; l < L; l += kRankBlock)
+    hist[l * nb + b] = lower((uint32_t)l + 1) - lower((uint32_t)l);
 }
 
 // Slot of every row: off[leader * nb + block] + rank in block.
@@ -298,134 +479,189 @@ k_place(const int32_t* __restrict__ owner, const int32_t* __restrict__ lrank,
   keys[pos] = o;
 }
 
-// A thread per chunk: the two longest runs of equal leader among the chunk's
-// slots (sorted keys), the earlier run first on equal lengths.
+// k_place for up to kPlaceScan (leader, block) pairs: every block scans the
+// counts itself in LDS (a thread per run of 32 counts, then a scan of the
+// 256 run totals) instead of a separate scan launch.
+constexpr int kPlaceScan = 8192;
 __global__ void __launch_bounds__(256)
-k_chunk_runs(const int* __restrict__ keys, int64_t M, int4* __restrict__ runs) {
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= ceil_div(M, kChunk)) return;
+k_place_scan(const int32_t* __restrict__ owner, const int32_t* __restrict__ lrank,
+             const int32_t* __restrict__ hist, int64_t pairs, int64_t M, int64_t nb,
+             int32_t* __restrict__ perm, int32_t* __restrict__ keys) {
+  __shared__ int32_t so[kPlaceScan];
+  __shared__ int32_t tot[256];
+  const int t = threadIdx.x;
+  for (int64_t e = t; e < pairs; e += 256) so[e] = hist[e];
+  __syncthreads();
+  const int per = (int)ceil_div(pairs, 256);
+  const int64_t a = (int64_t)t * per, z = std::min<int64_t>(a + per, pairs);
+  int32_t s = 0;
+  for (int64_t e = a; e < z; ++e) s += so[e];
+  tot[t] = s;
+  __syncthreads();
+  // inclusive scan of the 256 totals (Hillis-Steele)
+  for (int o = 1; o < 256; o <<= 1) {
+    const int32_t v = t >= o ? tot[t - o] : 0;
+    __syncthreads();
+    tot[t] += v;
+    __syncthreads();
+  }
+  int32_t run = t > 0 ? tot[t - 1] : 0;
+  for (int64_t e = a; e < z; ++e) {
+    const int32_t v = so[e];
+    so[e] = run;
+    run += v;
+  }
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * 256 + t;
+  if (r >= M) return;
+  const int32_t o = owner[r];
+  const int64_t pos = (int64_t)so[(int64_t)o * nb + r / kRankBlock] + lrank[r];
+  perm[pos] = (int32_t)r;
+  keys[pos] = o;
+}
+
+// ---------------------------------------------------------------------------
+// 4, 5. chunk bases and slot flags: a block per chunk of 16 slots
+// ---------------------------------------------------------------------------
+// The chunk's two longest runs of equal leader (the earlier on equal
+// lengths); each base's word w is the majority (count * 2 > n + 1) of its
+// run's rows, counted with 5-bit bit-sliced counters; then a wave per slot
+// compares popcount(row ^ base_j) with popcount(row) and flags the base with
+// the fewest bits when that is fewer than the row's own.
+__global__ void __launch_bounds__(256)
+k_chunk_plan(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+             const int32_t* __restrict__ perm, const int32_t* __restrict__ keys,
+             uint32_t* __restrict__ CM, int32_t* __restrict__ slot_row) {
+  __shared__ int4 srun;
+  __shared__ int32_t cp[kChunk], ck[kChunk];
+  const int64_t c = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
   const int n = (int)std::min<int64_t>(kChunk, M - kChunk * c);
-  const int* k = keys + kChunk * c;
-  int s0 = 0, n0 = 0, s1 = 0, n1 = 0;
-  for (int i = 0; i < n;) {
-    int j = i + 1;
-    while (j < n && k[j] == k[i]) ++j;
-    const int len = j - i;
-    if (len > n0) {
-      s1 = s0, n1 = n0;
-      s0 = i, n0 = len;
-    } else if (len > n1) {
-      s1 = i, n1 = len;
+  if (t < n) {
+    cp[t] = perm[kChunk * c + t];
+    ck[t] = keys[kChunk * c + t];
+  }
+  __syncthreads();
+  if (t == 0) {
+    int s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+    for (int i = 0; i < n;) {
+      int j = i + 1;
+      while (j < n && ck[j] == ck[i]) ++j;
+      const int len = j - i;
+      if (len > n0) {
+        s1 = s0, n1 = n0;
+        s0 = i, n0 = len;
+      } else if (len > n1) {
+        s1 = i, n1 = len;
+      }
+      i = j;
     }
-    i = j;
+    srun = make_int4(s0, n0, s1, n1);
   }
-  runs[c] = make_int4(s0, n0, s1, n1);
-}
-
-// A thread per (chunk, base, word): majority bits of the base's run.
-__global__ void __launch_bounds__(256)
-k_chunk_mask(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-             const int* __restrict__ perm, const int4* __restrict__ runs,
-             uint32_t* __restrict__ CM) {
+  __syncthreads();
+  const int4 rn = srun;
   const int64_t wpr = words_for(K);
-  const int64_t chunks = ceil_div(M, kChunk);
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= 2 * chunks * ldw) return;
-  const int64_t cb = t / ldw, w = t - cb * ldw;
-  const int64_t c = cb >> 1;
-  const int4 rn = runs[c];
-  const int start = (cb & 1) ? rn.z : rn.x, n = (cb & 1) ? rn.w : rn.y;
-  if (w >= wpr || n == 0) {
-    CM[t] = 0u;
-    return;
-  }
-  const uint32_t vb = valid_bits(w, K);
-  uint32_t cnt[32];
+  uint32_t* cm = CM + 2 * c * ldw;
+  for (int64_t e = t; e < 2 * ldw; e += 256) {
+    const int b = e >= ldw ? 1 : 0;
+    const int64_t w = e - b * ldw;
+    const int start = b ? rn.z : rn.x, len = b ? rn.w : rn.y;
+    uint32_t out = 0u;
+    if (w < wpr && len > 0) {
+      const uint32_t vb = valid_bits(w, K);
+      uint32_t xs[kChunk];
 #pragma unroll
-  for (int b = 0; b < 32; ++b) cnt[b] = 0;
-  for (int i = 0; i < n; ++i) {
-    const uint32_t x = W[(int64_t)perm[kChunk * c + start + i] * ldw + w] & vb;
+      for (int i = 0; i < kChunk; ++i)
+        xs[i] = i < len ? W[(int64_t)cp[start + (i < len ? i : 0)] * ldw + w] & vb : 0u;
+      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;  // bit-sliced count, up to 16
 #pragma unroll
-    for (int b = 0; b < 32; ++b) cnt[b] += (x >> b) & 1u;
-  }
-  uint32_t out = 0;
+      for (int i = 0; i < kChunk; ++i) {
+        uint32_t x = xs[i];
+        uint32_t y = c0 & x; c0 ^= x; x = y;
+        y = c1 & x; c1 ^= x; x = y;
+        y = c2 & x; c2 ^= x; x = y;
+        y = c3 & x; c3 ^= x; x = y;
+        c4 ^= x;
+      }
+      // count >= T with T = (len + 1) / 2 + 1, compared from the top bit
+      const uint32_t T = (uint32_t)(len + 1) / 2 + 1;
+      const uint32_t cs[5] = {c0, c1, c2, c3, c4};
+      uint32_t gt = 0u, eq = 0xFFFFFFFFu;
 #pragma unroll
-  for (int b = 0; b < 32; ++b) out |= (2 * cnt[b] > (uint32_t)n + 1 ? 1u : 0u) << b;
-  CM[t] = out;
-}
-
-// A wave per slot: the row, flagged with the base whose difference from the
-// row has the fewest bits when that is fewer than the row's own.
-__global__ void __launch_bounds__(256)
-k_slot_flags(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
-             const int* __restrict__ perm, const uint32_t* __restrict__ CM,
-             int32_t* __restrict__ slot_row) {
-  const int lane = threadIdx.x & 63;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (s >= M) return;
-  const int64_t wpr = words_for(K);
-  const int r = perm[s];
-  const uint32_t* a = W + (int64_t)r * ldw;
-  const uint32_t* m0 = CM + (s / kChunk) * 2 * ldw;
-  const uint32_t* m1 = m0 + ldw;
-  int pa = 0, p0 = 0, p1 = 0;
-  for (int64_t w = lane; w < wpr; w += 64) {
-    const uint32_t x = a[w] & valid_bits(w, K);
-    pa += __popc(x);
-    p0 += __popc(x ^ m0[w]);
-    p1 += __popc(x ^ m1[w]);
+      for (int q = 4; q >= 0; --q) {
+        if ((T >> q) & 1u) {
+          eq &= cs[q];
+        } else {
+          gt |= eq & cs[q];
+          eq &= ~cs[q];
+        }
+      }
+      out = gt | eq;
+    }
+    cm[e] = out;
   }
-  pa = wave_sum(pa);
-  p0 = wave_sum(p0);
-  p1 = wave_sum(p1);
-  int32_t flag = 0;
-  int best = pa;
-  if (p0 < best) flag = BWS_SLOT_DELTA, best = p0;
-  if (p1 < best) flag = BWS_SLOT_DELTA | BWS_SLOT_BASE1;
-  if (lane == 0) slot_row[s] = r | flag;
+  __syncthreads();  // the bases are read back by every wave
+  const uint32_t* m0 = cm;
+  const uint32_t* m1 = cm + ldw;
+  for (int j = t >> 6; j < n; j += 4) {
+    const int64_t s = kChunk * c + j;
+    const int32_t r = cp[j];
+    const uint32_t* a = W + (int64_t)r * ldw;
+    uint32_t pa = 0, p0 = 0, p1 = 0;
+    for (int64_t w = lane; w < wpr; w += 64) {
+      const uint32_t x = a[w] & valid_bits(w, K);
+      pa += __popc(x);
+      p0 += __popc(x ^ m0[w]);
+      p1 += __popc(x ^ m1[w]);
+    }
+    pa = wave_reduce(pa, OpAdd());
+    p0 = wave_reduce(p0, OpAdd());
+    p1 = wave_reduce(p1, OpAdd());
+    int32_t flag = 0;
+    uint32_t best = pa;
+    if (p0 < best) flag = BWS_SLOT_DELTA, best = p0;
+    if (p1 < best) flag = BWS_SLOT_DELTA | BWS_SLOT_BASE1;
+    if (lane == 0) slot_row[s] = r | flag;
+  }
 }
 
 struct ClusterWs {
-  uint32_t* candT;
   uint16_t* D;
   int32_t* lead_row;
-  uint32_t* leadT;
   int32_t* owner;
   int32_t* dist;
-  unsigned long long* far;
+  unsigned long long* part;
   int32_t* lrank;
   int32_t* hist;
   int32_t* off;
   void* scan_ws;
-  int32_t* keys_out;
+  int32_t* keys;
   int32_t* perm;
-  int4* runs;
-  int64_t c, nb;
+  int64_t c, nb, G;
   size_t total;
 };
 
-ClusterWs carve_cluster(void* base, int64_t M, int64_t K, int L) {
+ClusterWs carve_cluster(void* base, int64_t M, int L) {
   ClusterWs w{};
-  const int64_t wpr = words_for(K);
   w.c = candidates_for(M, L);
   w.nb = ceil_div(M, kRankBlock);
+  w.G = pass_blocks(M);
   const int32_t Lc = L - exact_rounds(L);
   const bool search = Lc <= kSearchMax;
+  const int64_t pairs = (int64_t)L * w.nb;
+  const bool big = pairs > kPlaceScan;
   Carve cv(base);
-  w.candT = cv.take<uint32_t>(search ? (size_t)(wpr * w.c) : 0);
   w.D = cv.take<uint16_t>(search ? (size_t)(w.c * w.c) : 0);
   w.lead_row = cv.take<int32_t>((size_t)L);
-  w.leadT = cv.take<uint32_t>((size_t)(wpr * Lc));
   w.owner = cv.take<int32_t>((size_t)M);
   w.dist = cv.take<int32_t>((size_t)M);
-  w.far = cv.take<unsigned long long>((size_t)kExactRounds + 1);
+  w.part = cv.take<unsigned long long>((size_t)((exact_rounds(L) + 1) * w.G));
   w.lrank = cv.take<int32_t>((size_t)M);
-  w.hist = cv.take<int32_t>((size_t)(L * w.nb));
-  w.off = cv.take<int32_t>((size_t)(L * w.nb) + 1);
-  w.scan_ws = cv.take<uint8_t>(scan_workspace_bytes(L * w.nb));
-  w.keys_out = cv.take<int32_t>((size_t)M);
+  w.hist = cv.take<int32_t>((size_t)pairs);
+  w.off = cv.take<int32_t>(big ? (size_t)pairs + 1 : 0);
+  w.scan_ws = cv.take<uint8_t>(big ? scan_workspace_bytes(pairs) : 0);
+  w.keys = cv.take<int32_t>((size_t)M);
   w.perm = cv.take<int32_t>((size_t)M);
-  w.runs = cv.take<int4>((size_t)ceil_div(M, kChunk));
   w.total = cv.used;
   return w;
 }
@@ -444,7 +680,7 @@ size_t bws_cluster_workspace_size(int64_t rows, int64_t cols, int32_t max_cluste
   if (rows <= 0 || rows >= BWS_SLOT_BASE1 || cols < 0 || max_clusters < 0) return 0;
   const int L = leaders_for(rows, max_clusters);
   if ((int64_t)L * ceil_div(rows, kRankBlock) > kMaxRankPairs) return 0;
-  return carve_cluster(nullptr, rows, cols, L).total + 256;
+  return carve_cluster(nullptr, rows, L).total + 256;
 }
 
 bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* slot_row,
@@ -462,19 +698,18 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
   if (!ws_ok(workspace, workspace_bytes, bws_cluster_workspace_size(M, K, max_clusters)))
     return BWS_EWORKSPACE;
   hipStream_t s = stream;
-  const ClusterWs w = carve_cluster(workspace, M, K, L);
-  const int64_t wpr = words_for(K);
+  const ClusterWs w = carve_cluster(workspace, M, L);
   auto grid = [](int64_t n, int per) { return dim3((unsigned)std::max<int64_t>(1, ceil_div(n, per))); };
   // 1. leaders: the first Lc by the candidate search (or the candidates)
   const int32_t R = exact_rounds(L), Lc = L - R;
   if (Lc <= kSearchMax) {
-    if (wpr > 0) {
-      hipLaunchKernelGGL(k_rows_t, grid(wpr * w.c, 256), dim3(256), 0, s, A->words, M, K, A->ldw,
-                         nullptr, w.c, w.candT);
-      BWS_TRY(launch_check());
-    }
-    hipLaunchKernelGGL(k_cand_dist, grid(w.c * ceil_div(w.c, 64), 4), dim3(256), 0, s, w.candT,
-                       wpr, w.c, w.D);
+    const bool vec = A->ldw % 4 == 0 && reinterpret_cast<uintptr_t>(A->words) % 16 == 0;
+    if (vec)
+      hipLaunchKernelGGL(k_cand_dist<true>, dim3((unsigned)w.c), dim3(256), 0, s, A->words, M, K,
+                         A->ldw, w.c, w.D);
+    else
+      hipLaunchKernelGGL(k_cand_dist<false>, dim3((unsigned)w.c), dim3(256), 0, s, A->words, M, K,
+                         A->ldw, w.c, w.D);
     BWS_TRY(launch_check());
     hipLaunchKernelGGL(k_leader_search, dim3(1), dim3(kSearchMax), 0, s, w.D, w.c, M, Lc,
                        w.lead_row);
@@ -482,45 +717,33 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
     hipLaunchKernelGGL(k_stride_leaders, grid(Lc, 256), dim3(256), 0, s, M, Lc, w.lead_row);
   }
   BWS_TRY(launch_check());
-  // 2. nearest of those leaders, then the exact rounds over all rows
-  if (wpr > 0) {
-    hipLaunchKernelGGL(k_rows_t, grid(wpr * Lc, 256), dim3(256), 0, s, A->words, M, K, A->ldw,
-                       w.lead_row, (int64_t)Lc, w.leadT);
-    BWS_TRY(launch_check());
-  }
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(w.far),
-                     (int64_t)2 * (kExactRounds + 1), 0);
-  BWS_TRY(launch_check());
-  hipLaunchKernelGGL(k_assign, grid(M, 4), dim3(256), 0, s, A->words, M, K, A->ldw, w.leadT, Lc,
-                     w.owner, w.dist, w.far);
+  // 2. nearest of those leaders, then the exact rounds over all rows; each
+  //    pass leaves one key per block for the next round's leader
+  hipLaunchKernelGGL(k_assign, dim3((unsigned)w.G), dim3(256), 0, s, A->words, M, K, A->ldw,
+                     w.lead_row, Lc, w.owner, w.dist, w.part);
   BWS_TRY(launch_check());
   for (int32_t it = 0; it < R; ++it) {
-    hipLaunchKernelGGL(k_exact_round, grid(M, 4), dim3(256), 0, s, A->words, M, K, A->ldw, Lc, it,
-                       w.far, w.owner, w.dist);
+    hipLaunchKernelGGL(k_exact_round, dim3((unsigned)w.G), dim3(256), 0, s, A->words, M, K, A->ldw,
+                       Lc, it, w.part + it * w.G, w.G, w.part + (it + 1) * w.G, w.owner, w.dist);
     BWS_TRY(launch_check());
   }
   // 3. stable order by leader
   const int64_t pairs = (int64_t)L * w.nb;
-  hipLaunchKernelGGL(k_fill_i32, grid(std::min<int64_t>(pairs, 1 << 24), 256), dim3(256), 0, s,
-                     w.hist, pairs, 0);
-  BWS_TRY(launch_check());
   hipLaunchKernelGGL(k_block_rank, dim3((unsigned)w.nb), dim3(kRankBlock), 0, s, w.owner, M, w.nb,
-                     w.lrank, w.hist);
+                     (int32_t)L, w.lrank, w.hist);
   BWS_TRY(launch_check());
-  BWS_TRY(launch_exclusive_scan_i32(w.hist, pairs, w.off, w.scan_ws, s));
-  hipLaunchKernelGGL(k_place, grid(M, 256), dim3(256), 0, s, w.owner, w.lrank, w.off, M, w.nb,
-                     w.perm, w.keys_out);
+  if (pairs <= kPlaceScan) {
+    hipLaunchKernelGGL(k_place_scan, grid(M, 256), dim3(256), 0, s, w.owner, w.lrank, w.hist,
+                       pairs, M, w.nb, w.perm, w.keys);
+  } else {
+    BWS_TRY(launch_exclusive_scan_i32(w.hist, pairs, w.off, w.scan_ws, s));
+    hipLaunchKernelGGL(k_place, grid(M, 256), dim3(256), 0, s, w.owner, w.lrank, w.off, M, w.nb,
+                       w.perm, w.keys);
+  }
   BWS_TRY(launch_check());
-  // 4, 5. chunk runs, bases, slot flags
-  const int64_t chunks = ceil_div(M, kChunk);
-  hipLaunchKernelGGL(k_chunk_runs, grid(chunks, 256), dim3(256), 0, s, w.keys_out, M, w.runs);
-  BWS_TRY(launch_check());
-  const int64_t threads = 2 * chunks * A->ldw;
-  hipLaunchKernelGGL(k_chunk_mask, grid(threads, 256), dim3(256), 0, s, A->words, M, K, A->ldw,
-                     w.perm, w.runs, chunk_mask);
-  BWS_TRY(launch_check());
-  hipLaunchKernelGGL(k_slot_flags, grid(M, 4), dim3(256), 0, s, A->words, M, K, A->ldw, w.perm,
-                     chunk_mask, slot_row);
+  // 4, 5. chunk bases and slot flags
+  hipLaunchKernelGGL(k_chunk_plan, dim3((unsigned)ceil_div(M, kChunk)), dim3(256), 0, s, A->words,
+                     M, K, A->ldw, w.perm, w.keys, chunk_mask, slot_row);
   return launch_check();
 }
 

@@ -367,32 +367,33 @@ k_mask_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   }
 }
 
-// Mask stream of a row-clustering plan: a thread per (chunk, stage).
+// Mask stream of a row-clustering plan: 16 threads per (chunk, stage)
+// record, thread j for slot j (its add and subtract masks); the record's
+// bases, flag word (bit j: slot j subtracts in this stage) and zero words are
+// written by its threads together, the chunk's base use and the flags
+// gathered with wave ballots (four records per wave).
 template <int MW>
 __global__ void __launch_bounds__(256)
 k_cluster_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
                  const int32_t* __restrict__ slot_row, const uint32_t* __restrict__ CM,
                  int64_t chunks, int64_t nst, uint32_t* __restrict__ S) {
   using R = Rec<true, MW>;
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= chunks * nst) return;
-  const int64_t c = t / nst, st = t - c * nst;
+  const int64_t rec = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int j = threadIdx.x & 15;
+  const int sh = threadIdx.x & 48;  // first lane of the record within the wave
+  const bool live = rec < chunks * nst;
+  const int64_t c = live ? rec / nst : 0, st = live ? rec - c * nst : 0;
   const int64_t wpr = words_for(K);
-  const bool real = c * kQR < M;  // chunks past the last row block's end are all zero
-  uint32_t* out = S + t * R::words;
+  const bool real = live && c * kQR < M;  // chunks past the last row block's end are all zero
+  const int64_t s = c * kQR + j;
+  const int32_t sr = real && s < M ? slot_row[s] : -1;
+  const bool delta = sr >= 0 && (sr & BWS_SLOT_DELTA);
+  const bool one = (sr & BWS_SLOT_BASE1) != 0;
   // which bases some slot of the chunk uses
-  bool used0 = false, used1 = false;
-  int32_t sr[kQR];
-#pragma unroll
-  for (int j = 0; j < kQR; ++j) {
-    const int64_t s = c * kQR + j;
-    sr[j] = s < M ? slot_row[s] : -1;
-    if (sr[j] >= 0 && (sr[j] & BWS_SLOT_DELTA)) {
-      if (sr[j] & BWS_SLOT_BASE1) used1 = true;
-      else used0 = true;
-    }
-  }
-  uint32_t flags = 0;
+  const bool used0 = ((__ballot(delta && !one) >> sh) & 0xFFFFull) != 0;
+  const bool used1 = ((__ballot(delta && one) >> sh) & 0xFFFFull) != 0;
+  uint32_t* out = S + rec * R::words;
+  uint32_t any_sub = 0;
 #pragma unroll
   for (int h = 0; h < MW; ++h) {
     const int64_t w = st * MW + h;
@@ -400,29 +401,32 @@ k_cluster_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
     const uint32_t vb = ok ? valid_lsb(w, K) : 0u;
     const uint32_t b0 = ok ? __brev(CM[(2 * c) * ldw + w]) & vb : 0u;
     const uint32_t b1 = ok ? __brev(CM[(2 * c + 1) * ldw + w]) & vb : 0u;
-    out[R::base + h] = used0 ? b0 : 0u;
-    out[R::base + MW + h] = used1 ? b1 : 0u;
-#pragma unroll
-    for (int j = 0; j < kQR; ++j) {
-      uint32_t add = 0, sub = 0;
-      if (sr[j] >= 0 && ok) {
-        const uint32_t x = __brev(W[(int64_t)(sr[j] & (BWS_SLOT_BASE1 - 1)) * ldw + w]) & vb;
-        if (sr[j] & BWS_SLOT_DELTA) {
-          const uint32_t b = (sr[j] & BWS_SLOT_BASE1) ? b1 : b0;
-          add = x & ~b;
-          sub = b & ~x;
-        } else {
-          add = x;
-        }
+    uint32_t add = 0, sub = 0;
+    if (sr >= 0 && ok) {
+      const uint32_t x = __brev(W[(int64_t)(sr & (BWS_SLOT_BASE1 - 1)) * ldw + w]) & vb;
+      if (delta) {
+        const uint32_t b = one ? b1 : b0;
+        add = x & ~b;
+        sub = b & ~x;
+      } else {
+        add = x;
       }
+    }
+    if (live) {
       out[j * MW + h] = add;
       out[R::sub + j * MW + h] = sub;
-      if (sub) flags |= 1u << j;
+      if (j == 0) {
+        out[R::base + h] = used0 ? b0 : 0u;
+        out[R::base + MW + h] = used1 ? b1 : 0u;
+      }
     }
+    any_sub |= sub;
   }
-  out[R::flags] = flags;
-#pragma unroll
-  for (int i = R::flags + 1; i < R::sub; ++i) out[i] = 0u;
+  const uint32_t flags = (uint32_t)((__ballot(any_sub != 0u) >> sh) & 0xFFFFull);
+  if (live) {
+    if (j == 0) out[R::flags] = flags;
+    for (int i = R::flags + 1 + j; i < R::sub; i += 16) out[i] = 0u;
+  }
 }
 
 template <int KS>
@@ -451,7 +455,7 @@ bws_status launch_cluster_stream_t(const bws_bitmap* A, const int32_t* slot_row,
                                    const uint32_t* chunk_mask, uint32_t* S, hipStream_t s) {
   const int64_t nst = ceil_div(A->cols, KS);
   const int64_t chunks = ceil_div(A->rows, kQRows) * kQW;
-  const int64_t n = chunks * nst;
+  const int64_t n = chunks * nst * 16;  // 16 threads per (chunk, stage) record
   if (ceil_div(n, 256) > kMaxGrid) return BWS_ECAPACITY;
   hipLaunchKernelGGL(k_cluster_stream<KS / 32>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
                      A->words, A->rows, A->cols, A->ldw, slot_row, chunk_mask, chunks, nst, S);
