@@ -22,11 +22,12 @@
 //     the candidates themselves;
 //  2. every row's nearest of those leaders (16 rows per block pass, the
 //     leaders' words staged in LDS, the rows' read with scalar loads), then
-//     R = min(8, L - 1) exact farthest-point rounds over all rows: each round
+//     R = min(4, L - 1) exact farthest-point rounds over all rows: each round
 //     is one launch that reduces the previous pass's per-block keys to its
 //     leader (no same-address atomics);
-//  3. rows stably ordered by nearest leader: per 1024-row block a bitonic
-//     sort in LDS gives each row its rank among the block's rows of the same
+//  3. rows stably ordered by nearest leader: per 256-row block a bitonic
+//     sort (shuffles within a wave, LDS across waves) gives each row its
+//     rank among the block's rows of the same
 //     leader and the block's count of every leader; an exclusive scan over
 //     (leader, block) gives every row its slot (done by each block of the
 //     placing kernel itself up to 8192 pairs);
@@ -48,7 +49,7 @@ namespace {
 constexpr int kChunk = 16;     // slots per chunk (rows per wave of the streamed product)
 constexpr int32_t kMaxLeaders = 65536;
 constexpr int kSearchMax = 256;  // candidates of the farthest-point search
-constexpr int kRankBlock = 1024; // rows per block of the stable ranking
+constexpr int kRankBlock = 256;  // rows per block of the stable ranking
 constexpr int kPadKey = kMaxLeaders;  // sorts after every leader
 
 int32_t leaders_for(int64_t rows, int32_t max_clusters) {
@@ -60,7 +61,7 @@ int32_t leaders_for(int64_t rows, int32_t max_clusters) {
 }
 
 // Leaders taken by exact farthest-point rounds over all rows (the last ones).
-constexpr int32_t kExactRounds = 8;
+constexpr int32_t kExactRounds = 4;
 int32_t exact_rounds(int32_t L) { return std::max(0, std::min(kExactRounds, L - 1)); }
 
 // Candidate rows of the search for the first L - exact_rounds(L) leaders
@@ -198,11 +199,28 @@ k_cand_dist(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw, i
 __global__ void __launch_bounds__(kSearchMax)
 k_leader_search(const uint16_t* __restrict__ D, int64_t c, int64_t M, int32_t L,
                 int32_t* __restrict__ lead_row) {
-  __shared__ uint16_t sD[kSearchMax * kSearchMax];
+  // the table and one spare 16-byte slot (the target of loads past its end)
+  __shared__ __align__(16) uint16_t sD[kSearchMax * kSearchMax + 8];
   __shared__ int32_t sidx[kSearchMax];
   const int64_t n = c * c, n8 = n / 8;
-  for (int64_t e = threadIdx.x; e < n8; e += kSearchMax)  // D is 256-byte aligned
-    reinterpret_cast<uint4*>(sD)[e] = reinterpret_cast<const uint4*>(D)[e];
+  // D is 256-byte aligned: 16-byte loads, eight in flight per thread (loads
+  // and stores unconditional, so that none waits for another)
+  const uint4* src = reinterpret_cast<const uint4*>(D);
+  uint4* dst = reinterpret_cast<uint4*>(sD);
+  constexpr int64_t kSpare = (int64_t)kSearchMax * kSearchMax / 8;
+  for (int64_t e0 = threadIdx.x; e0 < n8; e0 += 8 * kSearchMax) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t e = e0 + u * kSearchMax;
+      v[u] = src[e < n8 ? e : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t e = e0 + u * kSearchMax;
+      dst[e < n8 ? e : kSpare] = v[u];
+    }
+  }
   for (int64_t e = n8 * 8 + threadIdx.x; e < n; e += kSearchMax) sD[e] = D[e];
   __syncthreads();
   if (threadIdx.x >= 64) return;
@@ -270,23 +288,25 @@ k_assign(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
         const int n = (int)std::min<int64_t>(kLeadWords, wpr - w0);
         __syncthreads();  // slr written; the previous pass's readers are done
         {
+          // unconditional loads from clamped addresses (every one in flight
+          // at once), words past n or rows past M zeroed after
           const int k = t & 127, i0 = t >> 7;
+          const int kk = k < n ? k : 0;
           const uint32_t vb = k < n ? valid_bits(w0 + k, K) : 0u;
           uint32_t v[32];
 #pragma unroll
-          for (int s = 0; s < 32; ++s)
-            v[s] = k < n ? W[(int64_t)slr[i0 + 2 * s] * ldw + w0 + k] & vb : 0u;
+          for (int s = 0; s < 32; ++s) v[s] = W[(int64_t)slr[i0 + 2 * s] * ldw + w0 + kk];
 #pragma unroll
-          for (int s = 0; s < 32; ++s) sl[k * kLeadPitch + i0 + 2 * s] = v[s];
+          for (int s = 0; s < 32; ++s) sl[k * kLeadPitch + i0 + 2 * s] = v[s] & vb;
           uint32_t x[8];
 #pragma unroll
           for (int s = 0; s < 8; ++s) {
-            const int row = (t >> 7) + 2 * s;
-            const int64_t r = rb * kPassRows + row;
-            x[s] = (k < n && r < M) ? W[r * ldw + w0 + k] & vb : 0u;
+            const int64_t r = rb * kPassRows + i0 + 2 * s;
+            x[s] = W[(r < M ? r : M - 1) * ldw + w0 + kk];
           }
 #pragma unroll
-          for (int s = 0; s < 8; ++s) srw[(t >> 7) + 2 * s][k] = x[s];
+          for (int s = 0; s < 8; ++s)
+            srw[i0 + 2 * s][k] = rb * kPassRows + i0 + 2 * s < M ? x[s] & vb : 0u;
         }
         __syncthreads();
 #pragma unroll
@@ -352,9 +372,10 @@ k_exact_round(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   for (int q = 0; q < 4; ++q) {
     const int64_t r = std::min<int64_t>(rb0 * kPassRows + wave * 4 + q, M - 1);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < 2; ++u) {  // clamped load, then zeroed past the row
       const int64_t w = lane + 64 * u;
-      pre[q][u] = w < wpr ? W[r * ldw + w] & valid_bits(w, K) : 0u;
+      const uint32_t x = W[r * ldw + (w < wpr ? w : 0)];
+      pre[q][u] = w < wpr ? x & valid_bits(w, K) : 0u;
     }
   }
   uint64_t k0 = 0ull;
@@ -366,24 +387,34 @@ k_exact_round(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
     const int64_t r0 = rb * kPassRows + wave * 4;
     uint32_t d[4] = {0u, 0u, 0u, 0u};
     for (int64_t wb = 0; wb < wpr; wb += 128) {
+      // clamped loads, all in flight at once; words past the row count zero
       const bool first = rb == rb0 && wb == 0;
+      int64_t wc[2];
+      uint32_t vb[2], l[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int64_t w = wb + lane + 64 * u;
-        if (w < wpr) {
-          const uint32_t vb = valid_bits(w, K);
-          const uint32_t l = lw[w] & vb;
-          if (first) {
+        wc[u] = w < wpr ? w : 0;
+        vb[u] = w < wpr ? valid_bits(w, K) : 0u;
+        l[u] = lw[wc[u]];
+      }
+      if (first) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) d[q] += __popc(pre[q][u] ^ l);
-          } else {
+        for (int u = 0; u < 2; ++u)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int64_t r = r0 + q < M ? r0 + q : M - 1;
-              d[q] += __popc((W[r * ldw + w] & vb) ^ l);
-            }
-          }
+          for (int q = 0; q < 4; ++q) d[q] += __popc(pre[q][u] ^ (l[u] & vb[u]));
+      } else {
+        uint32_t x[4][2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t r = r0 + q < M ? r0 + q : M - 1;
+#pragma unroll
+          for (int u = 0; u < 2; ++u) x[q][u] = W[r * ldw + wc[u]];
         }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) d[q] += __popc((x[q][u] ^ l[u]) & vb[u]);
       }
     }
 #pragma unroll
@@ -409,55 +440,60 @@ k_exact_round(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
 // ---------------------------------------------------------------------------
 // 3. stable order by leader
 // ---------------------------------------------------------------------------
-// Stable ranking of a 1024-row block by leader: bitonic sort of (leader << 10
-// | row in block) in LDS; a row's rank is its sorted position minus the start
-// of its leader's run; the block's count of every leader l < L (zero when it
-// has none) goes to hist[l * nb + block], by binary search over the keys.
+// Stable ranking of a 256-row block by leader: bitonic sort of (leader << 8 |
+// row in block), a key per thread (partners within a wave exchanged by
+// shuffles, across waves through LDS); a row's rank is its sorted position
+// minus the start of its leader's run (a max-scan of run starts); the
+// block's count of every leader l < L (zero when it has none) goes to
+// hist[l * nb + block], by binary search over the sorted keys.
 __global__ void __launch_bounds__(kRankBlock)
 k_block_rank(const int32_t* __restrict__ owner, int64_t M, int64_t nb, int32_t L,
              int32_t* __restrict__ lrank, int32_t* __restrict__ hist) {
+  static_assert(kRankBlock == 256, "keys hold the row in block in 8 bits");
   __shared__ uint32_t sk[kRankBlock];
-  __shared__ int32_t ss[kRankBlock];
-  const int t = threadIdx.x;
+  __shared__ int32_t wmax[kRankBlock / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t b = blockIdx.x;
   const int64_t r = b * kRankBlock + t;
-  const uint32_t key = r < M ? (uint32_t)owner[r] : (uint32_t)kPadKey;
-  sk[t] = (key << 10) | (uint32_t)t;
-  __syncthreads();
+  uint32_t key = ((r < M ? (uint32_t)owner[r] : (uint32_t)kPadKey) << 8) | (uint32_t)t;
   for (int k = 2; k <= kRankBlock; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      const int p = t ^ j;
-      if (p > t) {
-        const uint32_t x = sk[t], y = sk[p];
-        const bool up = (t & k) == 0;
-        if ((x > y) == up) {
-          sk[t] = y;
-          sk[p] = x;
-        }
+      uint32_t other;
+      if (j >= 64) {
+        __syncthreads();  // the previous exchange is read
+        sk[t] = key;
+        __syncthreads();
+        other = sk[t ^ j];
+      } else {
+        other = (uint32_t)__shfl_xor((int)key, j, 64);
       }
-      __syncthreads();
+      // ascending runs: the lower position of a pair keeps the smaller key
+      const bool keep_min = ((t & j) == 0) == ((t & k) == 0);
+      key = keep_min ? (key < other ? key : other) : (key > other ? key : other);
     }
   }
-  // start of each position's run: inclusive max-scan of run starts
-  const uint32_t e = sk[t];
-  const bool head = t == 0 || (sk[t - 1] >> 10) != (e >> 10);
-  ss[t] = head ? t : 0;
   __syncthreads();
-  for (int o = 1; o < kRankBlock; o <<= 1) {
-    const int32_t v = t >= o ? ss[t - o] : 0;
-    __syncthreads();
-    ss[t] = v > ss[t] ? v : ss[t];
-    __syncthreads();
+  sk[t] = key;
+  __syncthreads();
+  // start of each position's run: inclusive max-scan of run starts
+  const bool head = t == 0 || (sk[t - 1] >> 8) != (key >> 8);
+  int32_t v = head ? t : 0;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v = u > v ? u : v;
   }
-  const int src = (int)(e & 1023u);
-  const int64_t row = b * kRankBlock + src;
-  if (row < M) lrank[row] = t - ss[t];
+  if (lane == 63) wmax[wave] = v;
+  __syncthreads();
+  for (int w = 0; w < wave; ++w) v = wmax[w] > v ? wmax[w] : v;
+  const int64_t row = b * kRankBlock + (int64_t)(key & 255u);
+  if (row < M) lrank[row] = t - v;
   // first position whose leader is >= x
   auto lower = [&](uint32_t x) {
     int lo = 0, hi = kRankBlock;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if ((sk[mid] >> 10) < x) lo = mid + 1;
+      if ((sk[mid] >> 8) < x) lo = mid + 1;
       else hi = mid;
     }
     return lo;
@@ -569,10 +605,11 @@ k_chunk_plan(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
     uint32_t out = 0u;
     if (w < wpr && len > 0) {
       const uint32_t vb = valid_bits(w, K);
-      uint32_t xs[kChunk];
+      uint32_t xs[kChunk];  // clamped loads all in flight, then zeroed past the run
 #pragma unroll
-      for (int i = 0; i < kChunk; ++i)
-        xs[i] = i < len ? W[(int64_t)cp[start + (i < len ? i : 0)] * ldw + w] & vb : 0u;
+      for (int i = 0; i < kChunk; ++i) xs[i] = W[(int64_t)cp[start + (i < len ? i : 0)] * ldw + w];
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) xs[i] = i < len ? xs[i] & vb : 0u;
       uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;  // bit-sliced count, up to 16
 #pragma unroll
       for (int i = 0; i < kChunk; ++i) {

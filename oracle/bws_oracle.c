@@ -345,8 +345,8 @@ int32_t oref_cluster_leaders(int64_t rows, int32_t max_clusters) {
 }
 
 /* Leaders taken by the exact farthest-point rounds over all rows: the last
- * min(8, L - 1) of L. */
-int32_t oref_cluster_exact_rounds(int32_t L) { return L - 1 < 8 ? (L > 0 ? L - 1 : 0) : 8; }
+ * min(4, L - 1) of L. */
+int32_t oref_cluster_exact_rounds(int32_t L) { return L - 1 < 4 ? (L > 0 ? L - 1 : 0) : 4; }
 
 /* Candidate rows of the leader search for the first Lc = L - exact rounds
  * leaders: Lc <= 256 searches min(rows, 256) candidates; more take the
@@ -363,7 +363,7 @@ int64_t oref_cluster_candidates(int64_t rows, int32_t L) {
  * source/src/main.cu:50-247; the row-to-group intent of backup/exp.cu:49-88)
  * so that a group's rows share work; this plan groups similar rows so that a
  * chunk's shared bits are summed once.
- *   1. Leaders.  R = min(8, L - 1) exact rounds, Lc = L - R.  c =
+ *   1. Leaders.  R = min(4, L - 1) exact rounds, Lc = L - R.  c =
  *      oref_cluster_candidates(rows, L) candidate rows, candidate j = row
  *      floor(j * rows / c).  Leaders 0 .. Lc-1: with Lc <= 256, farthest
  *      point over the candidates with the distance min(popcount(x ^ y),

@@ -109,13 +109,13 @@ def test_exact_clusters(oracle, rng, patterns, reps, cols):
 
 def rederive_owners(b, nl):
     """Leaders and nearest-leader owners restated from the rules: the last
-    R = min(8, L - 1) leaders by exact farthest point over all rows (lowest
+    R = min(4, L - 1) leaders by exact farthest point over all rows (lowest
     row on ties); the first Lc = L - R by farthest point over the candidates
     j * rows // c (distance clamped at 65535, lowest candidate on ties) when
     Lc <= 256, else the candidates themselves."""
     import oracle
     rows = b.shape[0]
-    R = min(8, nl - 1)
+    R = min(4, nl - 1)
     assert R == oracle.cluster_exact_rounds(nl)
     Lc = nl - R
     c = Lc if Lc > 256 else min(rows, 256)
@@ -212,10 +212,10 @@ def test_delta_algebra_fp64(oracle, bws):
 def test_search_distance_clamp(oracle):
     """Distances above 65535 tie in the candidate search.  Row 0 empty, row 1
     at distance 69990 and row 2 at 70000 from it, row 3 five bits, rows 4..11
-    disjoint blocks of 1000 + i bits.  With 11 leaders (3 searched, 8 exact
+    disjoint blocks of 1000 + i bits.  With 11 leaders (7 searched, 4 exact
     rounds) the search takes rows 0, 1 (the lower of the tied candidates) and
-    11; the exact rounds then rows 10 .. 4 and last row 2.  Unclamped, row 2
-    would lead second and row 1 last, swapping their slots."""
+    11 .. 7; the exact rounds then rows 6, 5, 4 and last row 2.  Unclamped,
+    row 2 would lead second and row 1 last, swapping their slots."""
     cols = 70000
     b = np.zeros((12, cols), bool)
     b[1, 10:] = True
@@ -234,8 +234,8 @@ def test_candidate_count(oracle):
     assert oracle.cluster_candidates(0, 0) == 0
     assert oracle.cluster_candidates(4096, 64) == 256
     assert oracle.cluster_candidates(100, 1) == 100
-    assert oracle.cluster_candidates(100000, 1000) == 992
-    assert [oracle.cluster_exact_rounds(L) for L in (0, 1, 2, 9, 10, 64)] == [0, 0, 1, 8, 8, 8]
+    assert oracle.cluster_candidates(100000, 1000) == 996
+    assert [oracle.cluster_exact_rounds(L) for L in (0, 1, 2, 9, 10, 64)] == [0, 0, 1, 4, 4, 4]
 
 
 def test_leader_count(oracle):
