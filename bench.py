@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--k", type=int, default=4096)
     ap.add_argument("--p", type=float, default=0.03125, help="prototype density")
+    ap.add_argument("--flip", type=float, default=0.02,
+                    help="bit-flip probability of the clustered mask (--p 0 --flip 0: all zero)")
     ap.add_argument("--iid", action="store_true", help="iid Bernoulli(p) mask instead of clustered")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -171,7 +173,7 @@ def main():
     if args.iid:
         W = bws.gen_bitmap_iid(args.seed, M, K, args.p)
     else:
-        W = bws.gen_bitmap_clustered(args.seed, M, K, 64, args.p, 0.02)
+        W = bws.gen_bitmap_clustered(args.seed, M, K, 64, args.p, args.flip)
     B = bws.gen_f32(args.seed + 1, (K, N), "normal", 0.0, 1.0)
     nnz = popcount_u32(W)
     A = bws.from_host_words(W, M, K, dev)
@@ -221,7 +223,8 @@ def main():
              "fp32": "k_spmm_stream<64, 2, false>"}.get(path)
     traffic = None
     tfile = PROFILE_TRAFFIC.format(path)
-    if os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125:
+    if (os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125
+            and args.flip == 0.02):
         try:
             tj = json.load(open(tfile))
             # only a counter reading of the kernel that runs now
