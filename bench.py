@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--quick", action="store_true", help="extras: skip the C4/C5/comparator legs")
+    ap.add_argument("--mfma-staging", default="auto", choices=["auto", "sparse"],
+                    help="16-bit precisions: dense MFMA (auto) or the sparse MFMA")
     ap.add_argument("--fp32-path", default="auto", choices=["auto", "clustered", "plain"],
                     help="FP32 kernel: the row-clustered product, the plain one, or by the plan")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -183,7 +185,9 @@ def main():
     C = torch.empty((M, N), dtype=torch.float32, device=dev)
     log(f"workload M=N=K={M}x{N}x{K} nnz={nnz} density={nnz / (M * K):.4f} prec={args.precision}")
 
-    plain = lambda: bws.spmm(A, Bin, args.precision, out=C)
+    mplan = (bws.make_plan(kernel=2, mfma_staging=args.mfma_staging)
+             if args.precision in ("bf16", "fp16") and args.mfma_staging != "auto" else None)
+    plain = lambda: bws.spmm(A, Bin, args.precision, out=C, plan=mplan)
     step, path, plan_info, clusters = plain, args.precision, None, None
     if args.precision == "fp32" and args.fp32_path != "plain":
         # the row-clustering plan: an analysis step per bitmap, timed apart
@@ -255,6 +259,8 @@ def main():
         tf = mfma_flops / (ms_kernel * 1e-3) / 1e12
         t_mfma_us = mfma_flops / (MFMA_PEAK_TFS * 1e12) * 1e6
         roofline = {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_PEAK_TFS,
+                    "mfma_path": "sparse (v_smfmac, dense-equivalent flops)"
+                    if mplan is not None else "dense",
                     "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFS, 4), "traffic": traffic,
                     "kernel_ms": round(ms_kernel, 5), "mfma_flops": mfma_flops,
                     "hbm_achieved_GBps": round(achieved, 1), "algorithmic_bytes": alg_bytes,
@@ -383,6 +389,13 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
             ms = tl(lambda: bws.spmm(A, Bx, name, out=C))
             legs[name] = {"fwd_ms": ms, "tflops": round(2.0 * nnz * N / ms / 1e9, 2),
                           "err_vs_fp64": ref.err(C)}
+        # BF16 on the sparse MFMA (4:2-compressed A): half the MFMA instructions
+        Bh = Bsrc.to(torch.bfloat16)
+        sp = bws.make_plan(kernel=2, mfma_staging="sparse")
+        ms = tl(lambda: bws.spmm(A, Bh, "bf16", out=C, plan=sp))
+        legs["bf16_sparse_mfma"] = {"fwd_ms": ms, "tflops": round(2.0 * nnz * N / ms / 1e9, 2),
+                                    "err_vs_fp64": ref.err(C)}
+        del Bh
         if clusters is not None:
             ms = tl(lambda: bws.spmm_clustered(A, clusters, Bsrc, out=C))
             legs["fp32_rowclusters"] = {"fwd_ms": ms,
@@ -436,7 +449,8 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
 def _density_sweep(bws, dev, M, N, K, args):
     """The clustered workload at nominal prototype densities 0.5 / 0.1 / 0.05 /
     0.01 (measured density p·0.98 + (1-p)·0.02): FP32 plain and row-clustered
-    (with its plan's delta-row fraction) and BF16 MFMA forward times."""
+    (with its plan's delta-row fraction) and BF16 forward times on the dense
+    and the sparse MFMA."""
     import torch
     out = []
     B = torch.from_numpy(bws.gen_f32(args.seed + 1, (K, N), "normal", 0.0, 1.0)).to(dev)
@@ -454,11 +468,14 @@ def _density_sweep(bws, dev, M, N, K, args):
         f32 = tl(lambda: bws.spmm(A, B, "fp32", out=C))
         f32c = tl(lambda: bws.spmm_clustered(A, pl, B, out=C))
         b16 = tl(lambda: bws.spmm(A, Bb, "bf16", out=C))
+        sp = bws.make_plan(kernel=2, mfma_staging="sparse")
+        b16s = tl(lambda: bws.spmm(A, Bb, "bf16", out=C, plan=sp))
         fl = 2.0 * nnz * N / 1e9
         out.append({"p_proto": p, "density": round(nnz / (M * K), 4), "nnz": nnz,
                     "fp32_ms": round(f32, 4), "fp32_tflops": round(fl / f32, 2),
                     "fp32_rowclusters_ms": round(f32c, 4), "delta_rows": round(delta, 3),
-                    "bf16_ms": round(b16, 4), "bf16_tflops": round(fl / b16, 2)})
+                    "bf16_ms": round(b16, 4), "bf16_tflops": round(fl / b16, 2),
+                    "bf16_sparse_mfma_ms": round(b16s, 4)})
     return out
 
 

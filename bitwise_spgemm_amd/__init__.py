@@ -661,7 +661,7 @@ def spmm_clustered(A: Bitmap, clusters: RowClusters, B: torch.Tensor, alpha: flo
 # ----------------------------------------------------------------------------
 # (3) grouped tile GEMM
 # ----------------------------------------------------------------------------
-MFMA_STAGING = {"auto": 0, "lut": 1, "regs": 2, "dma3": 3, "dma2": 4}
+MFMA_STAGING = {"auto": 0, "lut": 1, "regs": 2, "dma3": 3, "dma2": 4, "sparse": 5}
 
 
 def make_plan(kernel: int = 0, rows_per_wg: int = 0, waves_per_wg: int = 0, k_stage: int = 0,

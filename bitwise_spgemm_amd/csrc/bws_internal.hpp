@@ -67,6 +67,7 @@ struct MfmaTile {
   int32_t dma = 0;   // direct-to-LDS staging of 16-bit B
   int32_t nbuf = 2;  // LDS stages of the direct-to-LDS kernel
   int32_t ks = 64;   // K rows per stage of the direct-to-LDS kernel (64 or 128)
+  int32_t sparse = 0; // the sparse-MFMA kernel (spmm_smfmac.hip)
 };
 bool dma_ok(int32_t precision, int32_t b_dtype, const void* B, int64_t ldb, int64_t N);
 MfmaTile mfma_tile_for(const bws_plan* plan, int32_t precision, int32_t b_dtype, int64_t M,
@@ -84,6 +85,10 @@ struct BatchEntry {
   int64_t row_blocks, col_blocks;
 };
 
+// the sparse-MFMA kernel for a 0/1 A and 16-bit B (spmm_smfmac.hip)
+bws_status launch_smfmac(int32_t precision, const bws_bitmap* A, const void* B, int64_t ldb,
+                         int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                         hipStream_t s);
 // the 256 x 256-tile direct-to-LDS kernel (spmm_mfma_big.hip)
 bws_status launch_mfma4_big(int32_t precision, const bws_bitmap* A, const void* B, int64_t ldb,
                             int64_t N, float* C, int64_t ldc, float alpha, float beta,

@@ -687,6 +687,7 @@ MfmaTile mfma_tile_for(const bws_plan* plan, int32_t precision, int32_t b_dtype,
     t.legacy = 1;
     return t;
   }
+  t.sparse = variant == BWS_MFMA_SPARSE;
   // direct-to-LDS unless register staging is asked for; the caller clears it
   // when B does not allow it
   t.dma = variant != BWS_MFMA_REGS;
@@ -740,6 +741,10 @@ bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_
                      const bws_plan* plan, hipStream_t s) {
   MfmaTile t = mfma_tile_for(plan, precision, b_dtype, A->rows, N);
   if (!dma_ok(precision, b_dtype, B, ldb, N)) t.dma = 0;
+  if (t.sparse && t.dma && !A->values &&
+      ((precision == BWS_PREC_BF16 && b_dtype == BWS_DTYPE_BF16) ||
+       (precision == BWS_PREC_FP16 && b_dtype == BWS_DTYPE_F16)))
+    return launch_smfmac(precision, A, B, ldb, N, C, ldc, alpha, beta, s);
   if (!t.dma && t.bm == 256 && t.bn == 256) t.bm = 128;  // 256 x 256 needs the direct copy
 #define BWS_M(P, T) return launch_mfma_cfg<P, T>(t, A, B, ldb, N, C, ldc, alpha, beta, s)
   if (precision == BWS_PREC_BF16) {

@@ -323,14 +323,21 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
  *                 else it runs as 128 x 256; BF16X2 uses 128)
  *   mfma_staging  MFMA path: how B reaches LDS (BWS_MFMA_*; AUTO = direct-to-LDS
  *                 with 2 stages where B allows it: 16-bit B, 16-byte aligned
- *                 rows, N % 8 == 0; otherwise register staging)
+ *                 rows, N % 8 == 0; otherwise register staging).
+ *                 BWS_MFMA_SPARSE: a 0/1 A with 16-bit B that allows the
+ *                 direct copy runs on v_smfmac_f32_16x16x64 (A 4:2-compressed
+ *                 per row and 4 K columns; nibbles with 3-4 bits take a second
+ *                 sparse MFMA), half the MFMA instructions of the dense
+ *                 kernel, results equal to it up to FP32 summation order;
+ *                 other operands run as AUTO
  *   reserved      pass 0 */
 enum {
   BWS_MFMA_AUTO = 0,
   BWS_MFMA_LUT = 1,   /* 64 x 64 per wave, A fragments from an LDS table      */
   BWS_MFMA_REGS = 2,  /* B loaded to registers, transposed, written to LDS    */
   BWS_MFMA_DMA3 = 3,  /* B copied straight into LDS, 3 stages in flight       */
-  BWS_MFMA_DMA2 = 4   /* B copied straight into LDS, 2 stages (the default)   */
+  BWS_MFMA_DMA2 = 4,  /* B copied straight into LDS, 2 stages (the default)   */
+  BWS_MFMA_SPARSE = 5 /* 0/1 A on the sparse MFMA (4:2-compressed A, see below) */
 };
 typedef struct {
   int32_t kernel;
