@@ -122,7 +122,8 @@ _SIGS = {
     "bws_gather_f32": ([_P, _P, _I64, _P, _P], _I32),
     "bws_convert_workspace_size": ([_I64, _I64], _SZ),
     "bws_coo_to_bitmap": ([_P, _P, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P], _I32),
-    "bws_csr_to_bitmap": ([_P, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P], _I32),
+    "bws_csr_to_bitmap": ([_P, _P, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P],
+                          _I32),
     "bws_bitmap_to_csr": ([_BP, _P, _P, _P, _P, _SZ, _P], _I32),
     "bws_group_tiles": ([_I64, _I64, _I32], _I64),
     "bws_group_rows": ([_BP, _I32, _I32, C.POINTER(_Groups), _P], _I32),
@@ -150,7 +151,7 @@ _SIGS = {
                          _I32),
     "bws_batch_workspace_size": ([_I32], _SZ),
     "bws_batch_prepare": ([C.POINTER(_Problem), _I32, _I32, _I32, C.POINTER(_Plan), _P, _SZ,
-                           C.POINTER(_Batch)], _I32),
+                           C.POINTER(_Batch), _P], _I32),
     "bws_batch_run": ([C.POINTER(_Batch), _P], _I32),
     "bws_autotune": ([_BP, _I64, _P, _I32, _I64, _I64, _P, _I64, _I32, _I32, C.POINTER(_Plan),
                       C.POINTER(C.c_float), _P], _I32),
@@ -457,7 +458,7 @@ def csr_to_bitmap(row_ptr_t: torch.Tensor, col_idx: torch.Tensor, values: Option
         vout = torch.empty((max(nnz, 1),), dtype=torch.float32, device=dev)
     bad = torch.empty((1,), dtype=torch.int64, device=dev)
     ws = _workspace(lib().bws_convert_workspace_size(rows, cols), dev)
-    _check(lib().bws_csr_to_bitmap(_ptr(row_ptr_t), _ptr(col_idx), _ptr(values), rows, cols,
+    _check(lib().bws_csr_to_bitmap(_ptr(row_ptr_t), _ptr(col_idx), _ptr(values), nnz, rows, cols,
                                    _ptr(W), ldw, _ptr(rp), _ptr(vout), _ptr(bad), _ptr(ws),
                                    ws.numel(), _stream(dev)), "bws_csr_to_bitmap")
     A = Bitmap(W[:rows], rows, cols, rp, None if vout is None else vout[:nnz])
@@ -783,7 +784,8 @@ class Batch:
         self.batch = _Batch()
         _check(lib().bws_batch_prepare(self.arr, n, self.dtype, PRECISION[precision],
                                        None if plan is None else C.byref(plan), _ptr(self.ws),
-                                       self.ws.numel(), C.byref(self.batch)), "bws_batch_prepare")
+                                       self.ws.numel(), C.byref(self.batch),
+                                       _stream(self.device)), "bws_batch_prepare")
 
     @property
     def launches(self):

@@ -159,7 +159,7 @@ size_t bws_batch_workspace_size(int32_t count) {
 
 bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t b_dtype,
                              int32_t precision, const bws_plan* plan, void* workspace,
-                             size_t workspace_bytes, bws_batch* batch) {
+                             size_t workspace_bytes, bws_batch* batch, bws_stream stream) {
   if (!batch || count < 0 || (count > 0 && !problems)) return BWS_EINVAL;
   if (b_dtype < 0 || b_dtype > 2 || precision < 0 || precision > 3) return BWS_EINVAL;
   if (plan && (plan->kernel < 0 || plan->kernel > 2)) return BWS_EINVAL;
@@ -254,10 +254,15 @@ bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t
   int64_t* dev_offs = cv.take<int64_t>((size_t)(count + BWS_BATCH_MAX_LAUNCH + 1));
   (void)dev_offs;
   if (!entries.empty()) {
-    BWS_TRY(hip_check(hipMemcpy(dev_entries, entries.data(), entries.size() * sizeof(BatchEntry),
-                                hipMemcpyHostToDevice)));
-    BWS_TRY(hip_check(hipMemcpy(dev_offs, offs.data(), offs.size() * sizeof(int64_t),
-                                hipMemcpyHostToDevice)));
+    // copied on the caller's stream, after the work already queued there (the
+    // workspace may be memory that work still reads), then waited for: the
+    // host tables are freed on return
+    hipStream_t s = stream;
+    BWS_TRY(hip_check(hipMemcpyAsync(dev_entries, entries.data(),
+                                     entries.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, s)));
+    BWS_TRY(hip_check(hipMemcpyAsync(dev_offs, offs.data(), offs.size() * sizeof(int64_t),
+                                     hipMemcpyHostToDevice, s)));
+    BWS_TRY(hip_check(hipStreamSynchronize(s)));
   }
   return BWS_OK;
 }
@@ -309,7 +314,7 @@ bws_status bws_spmm_batched(const bws_problem* problems, int32_t count, int32_t 
                             size_t workspace_bytes, bws_stream stream) {
   bws_batch b;
   BWS_TRY(bws_batch_prepare(problems, count, b_dtype, precision, plan, workspace, workspace_bytes,
-                            &b));
+                            &b, stream));
   return bws_batch_run(&b, stream);
 }
 

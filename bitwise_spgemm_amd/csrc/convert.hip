@@ -59,24 +59,46 @@ k_coo_set(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci, int64_
     set_entry(ri[i], ci[i], M, K, W, ldw, bad);
 }
 
-// CSR: one wave per row.
-__global__ void __launch_bounds__(kBlock)
-k_csr_set(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, int64_t M, int64_t K,
-          uint32_t* __restrict__ W, int64_t ldw, unsigned long long* __restrict__ bad) {
-  const int lane = threadIdx.x & 63;
-  const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t r = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < M; r += waves)
-    for (int64_t i = rp[r] + lane; i < rp[r + 1]; i += 64) set_entry(r, ci[i], M, K, W, ldw, bad);
+// The entries row r of a CSR matrix may read: [rp[r], rp[r+1]) clamped to
+// [0, nnz) and to non-decreasing bounds.  A row pointer that is negative,
+// decreasing or past nnz (and rp[0] != 0) is an invalid input; the kernel
+// that sets the bits counts one per such row into *bad.
+__device__ __forceinline__ void csr_range(const int64_t* __restrict__ rp, int64_t r, int64_t nnz,
+                                          int64_t& lo, int64_t& hi, bool& ok) {
+  const int64_t a = rp[r], b = rp[r + 1];
+  ok = a >= 0 && b >= a && b <= nnz && (r != 0 || a == 0);
+  lo = a < 0 ? 0 : (a > nnz ? nnz : a);
+  hi = b < lo ? lo : (b > nnz ? nnz : b);
 }
 
+// CSR: one wave per row.
+__global__ void __launch_bounds__(kBlock)
+k_csr_set(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, int64_t nnz, int64_t M,
+          int64_t K, uint32_t* __restrict__ W, int64_t ldw, unsigned long long* __restrict__ bad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t r = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < M; r += waves) {
+    int64_t lo, hi;
+    bool ok;
+    csr_range(rp, r, nnz, lo, hi, ok);
+    if (!ok && lane == 0) atomicAdd(bad, 1ull);
+    for (int64_t i = lo + lane; i < hi; i += 64) set_entry(r, ci[i], M, K, W, ldw, bad);
+  }
+}
+
+// Value of entry (r, c) to its bitmap-order position in out [cap] (invalid
+// CSR row pointers can give more bits than entries: positions past cap are
+// not written).
 __device__ __forceinline__ void place(int64_t r, int64_t c, float v, int64_t M, int64_t K,
                                       const uint32_t* __restrict__ W, int64_t ldw, int64_t wpr,
                                       const int64_t* __restrict__ row_ptr,
-                                      const int32_t* __restrict__ woff, float* __restrict__ out) {
+                                      const int32_t* __restrict__ woff, float* __restrict__ out,
+                                      int64_t cap) {
   if (r < 0 || r >= M || c < 0 || c >= K) return;
   const int64_t w = c >> 5;
   const uint32_t before = W[r * ldw + w] & ~(0xFFFFFFFFu >> (c & 31));
-  out[row_ptr[r] + woff[r * wpr + w] + __popc(before)] = v;
+  const int64_t pos = row_ptr[r] + woff[r * wpr + w] + __popc(before);
+  if (pos < cap) out[pos] = v;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -87,19 +109,24 @@ k_coo_place(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci,
             float* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nnz; i += stride)
-    place(ri[i], ci[i], vals[i], M, K, W, ldw, wpr, row_ptr, woff, out);
+    place(ri[i], ci[i], vals[i], M, K, W, ldw, wpr, row_ptr, woff, out, nnz);
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_csr_place(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
-            const float* __restrict__ vals, int64_t M, int64_t K, const uint32_t* __restrict__ W,
-            int64_t ldw, int64_t wpr, const int64_t* __restrict__ row_ptr,
-            const int32_t* __restrict__ woff, float* __restrict__ out) {
+            const float* __restrict__ vals, int64_t nnz, int64_t M, int64_t K,
+            const uint32_t* __restrict__ W, int64_t ldw, int64_t wpr,
+            const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ woff,
+            float* __restrict__ out) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t r = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < M; r += waves)
-    for (int64_t i = rp[r] + lane; i < rp[r + 1]; i += 64)
-      place(r, ci[i], vals[i], M, K, W, ldw, wpr, row_ptr, woff, out);
+  for (int64_t r = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < M; r += waves) {
+    int64_t lo, hi;
+    bool ok;
+    csr_range(rp, r, nnz, lo, hi, ok);
+    for (int64_t i = lo + lane; i < hi; i += 64)
+      place(r, ci[i], vals[i], M, K, W, ldw, wpr, row_ptr, woff, out, nnz);
+  }
 }
 
 // One thread per word: column indices (and values) of its set bits.
@@ -205,12 +232,14 @@ bws_status bws_coo_to_bitmap(const int32_t* row_idx, const int32_t* col_idx, con
 }
 
 bws_status bws_csr_to_bitmap(const int64_t* csr_row_ptr, const int32_t* col_idx,
-                             const float* values, int64_t rows, int64_t cols, uint32_t* W,
-                             int64_t ldw, int64_t* row_ptr, float* values_out,
+                             const float* values, int64_t nnz, int64_t rows, int64_t cols,
+                             uint32_t* W, int64_t ldw, int64_t* row_ptr, float* values_out,
                              unsigned long long* n_invalid, void* workspace,
                              size_t workspace_bytes, bws_stream stream) {
-  if (!shape_ok(rows, cols, ldw) || !n_invalid) return BWS_EINVAL;
+  if (!shape_ok(rows, cols, ldw) || !n_invalid || nnz < 0) return BWS_EINVAL;
   if (rows > 0 && (!W || !csr_row_ptr)) return BWS_EINVAL;
+  if (nnz > 0 && !col_idx) return BWS_EINVAL;
+  if (values_out && nnz > 0 && !values) return BWS_EINVAL;
   if (!ws_ok(workspace, workspace_bytes, bws_convert_workspace_size(rows, cols)))
     return BWS_EWORKSPACE;
   hipStream_t s = stream;
@@ -220,7 +249,7 @@ bws_status bws_csr_to_bitmap(const int64_t* csr_row_ptr, const int32_t* col_idx,
   BWS_TRY(launch_check());
   if (rows > 0) {
     hipLaunchKernelGGL(k_csr_set, dim3(grid_of(rows * 64)), dim3(kBlock), 0, s, csr_row_ptr,
-                       col_idx, rows, cols, W, ldw, n_invalid);
+                       col_idx, nnz, rows, cols, W, ldw, n_invalid);
     BWS_TRY(launch_check());
   }
   const int64_t wpr = words_for(cols);
@@ -228,8 +257,8 @@ bws_status bws_csr_to_bitmap(const int64_t* csr_row_ptr, const int32_t* col_idx,
                      [&](const int64_t* rp, const int32_t* woff) {
                        if (rows > 0 && values)
                          hipLaunchKernelGGL(k_csr_place, dim3(grid_of(rows * 64)), dim3(kBlock),
-                                            0, s, csr_row_ptr, col_idx, values, rows, cols, W,
-                                            ldw, wpr, rp, woff, values_out);
+                                            0, s, csr_row_ptr, col_idx, values, nnz, rows, cols,
+                                            W, ldw, wpr, rp, woff, values_out);
                      },
                      s);
 }

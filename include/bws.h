@@ -184,15 +184,20 @@ bws_status bws_gather_f32(const float* src, const int64_t* idx, int64_t n, float
  * (row, column) order.  *n_invalid (device, written by the call) counts
  * entries outside the matrix and repeated positions; when it is nonzero the
  * bitmap holds the valid, distinct positions and values_out is unspecified at
- * repeated ones.  row_idx / col_idx are int32; CSR row pointers int64. */
+ * repeated ones (values_out is never written past entry nnz - 1).  row_idx /
+ * col_idx are int32; CSR row pointers int64. */
 size_t bws_convert_workspace_size(int64_t rows, int64_t cols);
 bws_status bws_coo_to_bitmap(const int32_t* row_idx, const int32_t* col_idx, const float* values,
                              int64_t nnz, int64_t rows, int64_t cols, uint32_t* W, int64_t ldw,
                              int64_t* row_ptr, float* values_out, unsigned long long* n_invalid,
                              void* workspace, size_t workspace_bytes, bws_stream stream);
+/* CSR: nnz = the length of col_idx and values.  Each row reads only entries
+ * in [0, nnz): a row pointer that is negative, decreasing, past nnz, or a
+ * nonzero csr_row_ptr[0], counts one into *n_invalid per row it affects and
+ * that row's range is clamped. */
 bws_status bws_csr_to_bitmap(const int64_t* csr_row_ptr, const int32_t* col_idx,
-                             const float* values, int64_t rows, int64_t cols, uint32_t* W,
-                             int64_t ldw, int64_t* row_ptr, float* values_out,
+                             const float* values, int64_t nnz, int64_t rows, int64_t cols,
+                             uint32_t* W, int64_t ldw, int64_t* row_ptr, float* values_out,
                              unsigned long long* n_invalid, void* workspace,
                              size_t workspace_bytes, bws_stream stream);
 /* Out of a bitmap: row_ptr [rows+1], col_idx [nnz] (ascending within a row,
@@ -467,7 +472,9 @@ bws_status bws_sddmm(const bws_bitmap* A, const void* dC, const void* B, int32_t
  * inside graph capture) validates the HOST array `problems`, picks each
  * problem's tile configuration (the plan's, or the autotuner's cached choice
  * for the problem's shape class, or the default), and writes the device
- * problem table and tile offsets into `workspace`.  bws_batch_run enqueues the
+ * problem table and tile offsets into `workspace` on `stream` (after the work
+ * already queued there, which may still use that memory; prepare returns
+ * when the copies are done).  bws_batch_run enqueues the
  * launches on `stream` and is capturable into a hipGraph; the workspace must
  * stay untouched between the two.  Workgroups find their problem by a binary
  * search over the tile offsets, so problems of very different sizes share a
@@ -513,7 +520,7 @@ typedef struct {
 size_t bws_batch_workspace_size(int32_t count);
 bws_status bws_batch_prepare(const bws_problem* problems, int32_t count, int32_t b_dtype,
                              int32_t precision, const bws_plan* plan, void* workspace,
-                             size_t workspace_bytes, bws_batch* batch);
+                             size_t workspace_bytes, bws_batch* batch, bws_stream stream);
 bws_status bws_batch_run(const bws_batch* batch, bws_stream stream);
 
 /* Convenience: prepare + run in one call (not capturable: prepare copies). */

@@ -134,14 +134,14 @@ def test_sddmm_and_groups_validation(bws):
     assert L.bws_spmm_batched(None, 3, 0, 0, None, None, 0, None) == bws.BWS_EINVAL
     assert L.bws_spmm_batched(None, 0, 0, 0, None, FAKE, L.bws_batch_workspace_size(0), None) == bws.BWS_OK
     b = bws._Batch()
-    assert L.bws_batch_prepare(None, 0, 0, 0, None, FAKE, 4096, C.byref(b)) == bws.BWS_OK
+    assert L.bws_batch_prepare(None, 0, 0, 0, None, FAKE, 4096, C.byref(b), None) == bws.BWS_OK
     assert b.n_launch == 0 and L.bws_batch_run(C.byref(b), None) == bws.BWS_OK
-    assert L.bws_batch_prepare(None, 0, 0, 0, None, None, 0, C.byref(b)) == bws.BWS_EWORKSPACE
-    assert L.bws_batch_prepare(None, 0, 1, 3, None, FAKE, 4096, C.byref(b)) == bws.BWS_EUNSUPPORTED
+    assert L.bws_batch_prepare(None, 0, 0, 0, None, None, 0, C.byref(b), None) == bws.BWS_EWORKSPACE
+    assert L.bws_batch_prepare(None, 0, 1, 3, None, FAKE, 4096, C.byref(b), None) == bws.BWS_EUNSUPPORTED
     arr = (bws._Problem * 1)()
     arr[0] = bws._Problem(_bm(64, 64, 1), FAKE, 64, 64, FAKE, 64, 0)   # ldw too small
     ws = L.bws_batch_workspace_size(1)
-    assert L.bws_batch_prepare(arr, 1, 0, 0, None, FAKE, ws, C.byref(b)) == bws.BWS_EINVAL
+    assert L.bws_batch_prepare(arr, 1, 0, 0, None, FAKE, ws, C.byref(b), None) == bws.BWS_EINVAL
 
 
 def test_workspace_sizes(bws):

@@ -110,3 +110,34 @@ def test_matrix_market_spmm(oracle, bws, dev, name):
     vals = A.values.cpu().numpy()
     C = bws.spmm(A, to_dev(B, dev), "fp32").cpu().numpy()
     assert rel_err(oracle, C, W, cols, B, rp, vals)["max_rel_s"] <= 1e-5
+
+
+@pytest.mark.parametrize("rp", [[0, 2, 1, 3, 5, 6],    # decreasing pointer (row 1)
+                                [0, 2, 3, 3, 5, 9],    # past nnz (row 4)
+                                [1, 2, 3, 3, 5, 6],    # nonzero first pointer (row 0)
+                                [0, -1, 3, 3, 5, 6]])  # negative pointer (rows 0 and 1)
+def test_csr_bad_row_pointers(bws, dev, rp):
+    """Invalid CSR row pointers: each affected row counts one into n_invalid
+    and reads only its range clamped to [0, nnz) and made non-decreasing;
+    nothing outside col_idx is read."""
+    M, K = 5, 40
+    cols = np.array([1, 7, 3, 30, 12, 39], np.int32)  # distinct columns
+    vals = np.arange(1, 7, dtype=np.float32)
+    nnz = len(cols)
+    A, bad = bws.csr_to_bitmap(to_dev(np.array(rp, np.int64), dev), to_dev(cols, dev),
+                               to_dev(vals, dev), (M, K))
+    expect = np.zeros((M, K), bool)
+    n_bad = 0
+    for r in range(M):
+        a, b = rp[r], rp[r + 1]
+        n_bad += not (a >= 0 and b >= a and b <= nnz and (r != 0 or a == 0))
+        lo = min(max(a, 0), nnz)
+        hi = min(max(b, lo), nnz)
+        expect[r, cols[lo:hi]] = True
+    assert int(bad.item()) == n_bad > 0
+    W = to_host_u32(A.words)
+    got = np.zeros((M, K), bool)
+    for r in range(M):
+        for c in range(K):
+            got[r, c] = (W[r, c // 32] >> (31 - c % 32)) & 1
+    assert np.array_equal(got, expect)
