@@ -449,8 +449,9 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
 def _density_sweep(bws, dev, M, N, K, args):
     """The clustered workload at nominal prototype densities 0.5 / 0.1 / 0.05 /
     0.01 (measured density p·0.98 + (1-p)·0.02): FP32 plain and row-clustered
-    (with its plan's delta-row fraction) and BF16 forward times on the dense
-    and the sparse MFMA."""
+    (with its plan's delta-row fraction), FP32 subset sums, and BF16 forward
+    times on the dense and the sparse MFMA; then an iid 50% mask (no row
+    structure) with FP32 plain, FP32 subset sums and BF16."""
     import torch
     out = []
     B = torch.from_numpy(bws.gen_f32(args.seed + 1, (K, N), "normal", 0.0, 1.0)).to(dev)
@@ -470,12 +471,27 @@ def _density_sweep(bws, dev, M, N, K, args):
         b16 = tl(lambda: bws.spmm(A, Bb, "bf16", out=C))
         sp = bws.make_plan(kernel=2, mfma_staging="sparse")
         b16s = tl(lambda: bws.spmm(A, Bb, "bf16", out=C, plan=sp))
+        ss = bws.make_plan(kernel=1, k_stage=16)
+        f32s = tl(lambda: bws.spmm(A, B, "fp32", out=C, plan=ss))
         fl = 2.0 * nnz * N / 1e9
         out.append({"p_proto": p, "density": round(nnz / (M * K), 4), "nnz": nnz,
                     "fp32_ms": round(f32, 4), "fp32_tflops": round(fl / f32, 2),
                     "fp32_rowclusters_ms": round(f32c, 4), "delta_rows": round(delta, 3),
                     "bf16_ms": round(b16, 4), "bf16_tflops": round(fl / b16, 2),
-                    "bf16_sparse_mfma_ms": round(b16s, 4)})
+                    "bf16_sparse_mfma_ms": round(b16s, 4), "fp32_subset_sums_ms": round(f32s, 4)})
+    # iid 50%: no row structure for the clustering; the subset-sum kernel's case
+    W = bws.gen_bitmap_iid(args.seed, M, K, 0.5)
+    nnz = popcount_u32(W)
+    A = bws.from_host_words(W, M, K, dev)
+    ss = bws.make_plan(kernel=1, k_stage=16)
+    f32 = tl(lambda: bws.spmm(A, B, "fp32", out=C))
+    f32s = tl(lambda: bws.spmm(A, B, "fp32", out=C, plan=ss))
+    b16 = tl(lambda: bws.spmm(A, Bb, "bf16", out=C))
+    fl = 2.0 * nnz * N / 1e9
+    out.append({"iid": 0.5, "density": round(nnz / (M * K), 4), "nnz": nnz,
+                "fp32_ms": round(f32, 4), "fp32_tflops": round(fl / f32, 2),
+                "fp32_subset_sums_ms": round(f32s, 4), "fp32_subset_sums_tflops": round(fl / f32s, 2),
+                "bf16_ms": round(b16, 4)})
     return out
 
 

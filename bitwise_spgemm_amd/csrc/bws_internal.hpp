@@ -109,6 +109,14 @@ bws_status launch_cluster_stream(const bws_bitmap* A, const int32_t* slot_row,
 bws_status launch_spmm_stream(const uint32_t* S, const int32_t* slot_row, int64_t M, int64_t K,
                               const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
                               float alpha, float beta, hipStream_t s);
+// The subset-sum product for dense masks (16-row sub-stages, 4-row groups):
+// its 32-row mask stream and the product (mask stream pass included).
+size_t subset_stream_bytes(int64_t M, int64_t K);
+bws_status launch_spmm_subset(const bws_bitmap* A, uint32_t* S, const float* B, int64_t ldb,
+                              int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                              hipStream_t s);
+// K rows per sub-stage of the subset-sum product (its plan: kernel 1, k_stage 16)
+constexpr int32_t kSubsetKStage = 16;
 // Rows and columns of C per workgroup of the streamed product.
 constexpr int64_t kStreamTileRows = 256;
 constexpr int64_t kStreamTileCols = 256;

@@ -161,6 +161,9 @@ size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision
                                const bws_plan* plan) {
   (void)N;
   if (!A || A->rows < 0 || A->cols < 0 || N < 0) return 0;
+  if (precision == BWS_PREC_FP32 && !A->values && N % 4 == 0 && plan && plan->kernel == 1 &&
+      plan->rows_per_wg == 0 && plan->k_stage == kSubsetKStage)
+    return subset_stream_bytes(A->rows, A->cols);
   if (precision == BWS_PREC_FP32 && !A->values && N % 4 == 0 && (!plan || plan->rows_per_wg == 0))
     return mask_stream_bytes(A->rows, A->cols);
   if (plan && plan->kernel == 1) return 0;
@@ -183,6 +186,8 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   BWS_TRY(check_bitmap(A, false));
   if (N < 0 || !dtype_ok(b_dtype) || !prec_ok(precision)) return BWS_EINVAL;
   if (!combo_ok(precision, b_dtype)) return BWS_EUNSUPPORTED;
+  const bool subset = plan && plan->kernel == 1 && plan->rows_per_wg == 0 &&
+                      plan->k_stage == kSubsetKStage;
   if (plan && (plan->kernel < 0 || plan->kernel > 2 || plan->rows_per_wg < 0 ||
                (plan->kernel != 2 && plan->rows_per_wg != 0 &&
                 !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N, A->values != nullptr))) ||
@@ -194,6 +199,14 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   if ((reinterpret_cast<uintptr_t>(C) & 3u) ||
       (reinterpret_cast<uintptr_t>(B) & (esize(b_dtype) - 1)))
     return BWS_EINVAL;
+  // FP32 on a 0/1 A, the plan asking for the subset-sum kernel (dense masks)
+  // and its mask stream in the workspace
+  if (subset && precision == BWS_PREC_FP32 && !A->values && A->cols > 0 &&
+      b_dtype == BWS_DTYPE_F32 && N >= 4 && N % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(B) % 16 == 0 && reinterpret_cast<uintptr_t>(C) % 16 == 0 &&
+      ws_ok(workspace, workspace_bytes, subset_stream_bytes(A->rows, A->cols)))
+    return launch_spmm_subset(A, static_cast<uint32_t*>(workspace), static_cast<const float*>(B),
+                              ldb, N, C, ldc, alpha, beta, stream);
   // FP32 on a 0/1 A with the mask stream in the workspace: the streamed
   // kernel (row masks walked on the scalar unit), for problems that fill the
   // chip with 256 x 256 tiles

@@ -336,6 +336,131 @@ k_spmm_stream(const uint32_t* __restrict__ S, const int32_t* __restrict__ slot_r
   }
 }
 
+// Subset-sum variant for dense masks (the bit-wise subset sums of
+// backup/bit_wise_gemm.cu:35-120, restated for LDS): K in sub-stages of 16
+// rows, 4 groups of 4.  Per sub-stage an LDS table holds, per group, the sums
+// of the group's B rows over all 16 subsets (entry e = sum of rows 4g + j for
+// the bits j of e, entry 0 a zero row), built from the sub-stage's B rows, so
+// a row's nibble of the group costs one LDS read and four adds whatever its
+// bits.  B rows come HBM -> LDS by global_load_lds_dwordx4 into a ring of RB
+// sub-stages (RB - 1 ahead); the masks are the 32-row mask stream (a record
+// per two sub-stages, the next one loaded while the current is used).
+constexpr int kSubRows = 16;
+constexpr int kSubRing = 4;
+__constant__ uint8_t kComposite[11] = {3, 5, 6, 7, 9, 10, 11, 12, 13, 14, 15};
+
+__global__ void __launch_bounds__(kQW * 64)
+k_spmm_subset(const uint32_t* __restrict__ S, int64_t M, int64_t nst, const float* __restrict__ B,
+              int64_t ldb, int64_t K, int64_t N, float* __restrict__ C, int64_t ldc, float alpha,
+              float beta, int64_t row_blocks, int64_t col_blocks, int64_t per_xcd) {
+  using R = Rec<false, 1>;
+  constexpr int RB = kSubRing;
+  __shared__ f32x4 raw[RB][kSubRows][64];  // 64 KiB
+  __shared__ f32x4 tab[4][16][64];         // [group][subset][lane]: 64 KiB
+
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t cb = tile / row_blocks, rb = tile - cb * row_blocks;
+  const int64_t n0 = cb * kQBN;
+  const int64_t chunk = rb * kQW + wave;
+  const int64_t ncol = (n0 + 4 * lane + 4 <= N) ? n0 + 4 * lane : N - 4;  // clamped column
+  const float* bcol = B + ncol;
+  const sgpr_src rec = (sgpr_src)(S + chunk * nst * R::words);
+  const int64_t nsub = ceil_div(K, kSubRows);
+
+  if (wave < 4) tab[wave][0][lane] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // wave w copies row 16 s + w of sub-stage s into raw[s % RB][w]
+  const uint32_t dst0 = lds_u32(&raw[0][wave][0]);
+  auto issue = [&](int64_t sub) {
+    int64_t k = sub * kSubRows + wave;
+    k = k < K ? k : K - 1;  // rows past K meet zero mask bits
+    dma16(bcol + k * ldb, dst0 + (uint32_t)(sub % RB) * (kSubRows * 1024u));
+  };
+
+  float4 acc[kQR];
+#pragma unroll
+  for (int r = 0; r < kQR; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t mc[kQR], mn[kQR];
+#pragma unroll
+  for (int r = 0; r < kQR; ++r) mc[r] = rec[r];
+#pragma unroll
+  for (int j = 0; j < RB - 1; ++j)
+    if (j < nsub) issue(j);
+  for (int64_t sub = 0; sub < nsub; ++sub) {
+    // this wave's copy of sub-stage sub has landed (younger ones may be in flight)
+    const int64_t younger = nsub - 1 - sub;
+    if (younger >= RB - 2) wait_vmcnt<RB - 2>();
+    else if (younger == 1) wait_vmcnt<1>();
+    else wait_vmcnt<0>();
+    s_barrier();  // every copy of sub landed; the table of sub - 1 is no longer read
+    if (sub + RB - 1 < nsub) issue(sub + RB - 1);
+    if ((sub & 1) == 0 && (sub >> 1) + 1 < nst) {  // the next record's masks
+      const sgpr_src p = rec + ((sub >> 1) + 1) * R::words;
+#pragma unroll
+      for (int r = 0; r < kQR; ++r) mn[r] = p[r];
+    }
+    // the table of this sub-stage: singles copied, composites summed
+    const int slot = (int)(sub % RB);
+    for (int t = wave; t < 60; t += kQW) {
+      const int g = t / 15, e = t - 15 * g;  // e: 0..3 singles (1 << e), 4..14 composites
+      f32x4 v;
+      int ent;
+      if (e < 4) {
+        ent = 1 << e;
+        v = raw[slot][4 * g + e][lane];
+      } else {
+        ent = kComposite[e - 4];
+        int rest = ent;
+        v = raw[slot][4 * g + __builtin_ctz(rest)][lane];
+        rest &= rest - 1;
+        while (rest) {
+          v += raw[slot][4 * g + __builtin_ctz(rest)][lane];
+          rest &= rest - 1;
+        }
+      }
+      tab[g][ent][lane] = v;
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");  // table complete
+    const int sh = (int)(sub & 1) * kSubRows;
+#pragma unroll
+    for (int r = 0; r < kQR; ++r) {
+      const uint32_t m16 = (mc[r] >> sh) & 0xFFFFu;
+      if (m16 != 0u) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) acc_add<false>(acc[r], tab[g][(m16 >> (4 * g)) & 15u][lane]);
+      }
+    }
+    if (sub & 1) {
+#pragma unroll
+      for (int r = 0; r < kQR; ++r) mc[r] = mn[r];
+    }
+  }
+
+  const int64_t n = n0 + 4 * lane;
+  if (n >= N) return;
+#pragma unroll
+  for (int r = 0; r < kQR; ++r) {
+    const int64_t row = chunk * kQR + r;
+    if (row >= M) break;
+    float4* c = reinterpret_cast<float4*>(C + row * ldc + n);
+    float4 v = acc[r];
+    v.x *= alpha;
+    v.y *= alpha;
+    v.z *= alpha;
+    v.w *= alpha;
+    if (beta != 0.0f) {
+      const float4 o = *c;
+      v.x = fmaf(beta, o.x, v.x);
+      v.y = fmaf(beta, o.y, v.y);
+      v.z = fmaf(beta, o.z, v.z);
+      v.w = fmaf(beta, o.w, v.w);
+    }
+    *c = v;
+  }
+}
+
 // Bits of word w (LSB-first after reversal) that are columns < cols.
 __device__ __forceinline__ uint32_t valid_lsb(int64_t w, int64_t cols) {
   const int64_t rem = cols - 32 * w;
@@ -485,6 +610,20 @@ bws_status launch_spmm_stream_t(const uint32_t* S, const int32_t* slot_row, int6
   return launch_check();
 }
 
+// The subset-sum product over the 32-row mask stream of launch_mask_stream_t<32>.
+bws_status launch_spmm_subset_t(const uint32_t* S, int64_t M, int64_t K, const float* B,
+                                int64_t ldb, int64_t N, float* C, int64_t ldc, float alpha,
+                                float beta, hipStream_t s) {
+  const int64_t nst = ceil_div(K, 32);
+  const int64_t row_blocks = ceil_div(M, kQRows);
+  const int64_t col_blocks = ceil_div(N, kQBN);
+  const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
+  if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
+  hipLaunchKernelGGL(k_spmm_subset, dim3((unsigned)(8 * per_xcd)), dim3(kQW * 64), 0, s, S, M,
+                     nst, B, ldb, K, N, C, ldc, alpha, beta, row_blocks, col_blocks, per_xcd);
+  return launch_check();
+}
+
 }  // namespace
 
 #ifndef BWS_STREAM_NO_PRODUCT
@@ -508,6 +647,13 @@ bws_status launch_spmm_stream(const uint32_t* S, const int32_t* slot_row, int64_
                               float alpha, float beta, hipStream_t s) {
   return launch_spmm_stream_t<kStreamKS, kStreamNB>(S, slot_row, M, K, B, ldb, N, C, ldc, alpha,
                                                     beta, s);
+}
+size_t subset_stream_bytes(int64_t M, int64_t K) { return mask_stream_bytes_t<32>(M, K); }
+bws_status launch_spmm_subset(const bws_bitmap* A, uint32_t* S, const float* B, int64_t ldb,
+                              int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                              hipStream_t s) {
+  BWS_TRY(launch_mask_stream_t<32>(A, S, s));
+  return launch_spmm_subset_t(S, A->rows, A->cols, B, ldb, N, C, ldc, alpha, beta, s);
 }
 #endif
 

@@ -321,7 +321,12 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
  *                 MFMA path: rows of C per workgroup (0 = auto, 128 or 256)
  *   waves_per_wg  vector-add path: 8 or 16 (0 = 16) (MFMA: always 4, pass 0)
  *   k_stage       vector-add path: K rows staged per LDS buffer (32 or 64)
- *                 (MFMA: fixed by the tile, pass 0)
+ *                 (MFMA: fixed by the tile, pass 0).  16 with kernel 1 and
+ *                 rows_per_wg 0: the FP32 subset-sum product for dense 0/1
+ *                 masks (per 4 K rows an LDS table of the 16 sums of their B
+ *                 rows; one read and four adds per row nibble, whatever its
+ *                 bits: faster than the bit walk above about 40% density),
+ *                 with the workspace bws_spmm_workspace_size reports for it
  *   mfma_min_bits reserved for the density classifier (0 = default)
  *   cols_per_wg   MFMA path: columns of C per workgroup (0 = auto, 128 or 256;
  *                 256 x 256 only with the direct-to-LDS staging of 16-bit B,
