@@ -441,6 +441,7 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     if not args.quick:
         ex["density_sweep"] = _density_sweep(bws, dev, M, N, K, args)
         ex["value_mask"] = _value_mask_workload(bws, dev)
+        ex["int8_bitplanes"] = _bitplanes_workload(bws, dev, Bd, M, N, K)
         ex["grouped"] = _grouped_workload(bws, dev)
         ex["comparators"] = _comparators(bws, A, W, Bd, M, N, K, nnz, dev)
     return ex
@@ -492,6 +493,37 @@ def _density_sweep(bws, dev, M, N, K, args):
                 "fp32_ms": round(f32, 4), "fp32_tflops": round(fl / f32, 2),
                 "fp32_subset_sums_ms": round(f32s, 4), "fp32_subset_sums_tflops": round(fl / f32s, 2),
                 "bf16_ms": round(b16, 4)})
+    return out
+
+
+def _bitplanes_workload(bws, dev, Bd, M, N, K):
+    """int8 A (M x K, 10% nonzero, values uniform over [-128, 127] without 0)
+    held as 8 packed bit-planes, times B: B read once (the planes folded into
+    one value-carrying or dense 16-bit A) against eight weighted 0/1 passes,
+    FP32 (FP32 B) and BF16 (BF16 B); errors against FP64 on sampled rows."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(7)
+    Ai = rng.integers(1, 128, size=(M, K), dtype=np.int16) * rng.choice([-1, 1], size=(M, K))
+    Ai[rng.random((M, K)) >= 0.1] = 0
+    Ai = np.clip(Ai, -128, 127).astype(np.int8)
+    P = bws.pack_bitplanes_i8(torch.from_numpy(Ai).to(dev))
+    rows = sample_rows(M)
+    Bb = Bd.to(torch.bfloat16)
+    C = torch.empty((M, N), device=dev)
+    out = {"workload": f"int8 A {M}x{K} (10% nonzero) as 8 bit-planes @ {K}x{N}"}
+    def tl(f):
+        spin_up(f, 0.05)
+        return round(time_loop(f, 10, 2)[1], 4)
+    for prec, Bx in (("fp32", Bd), ("bf16", Bb)):
+        ref = Fp64Ref(None, K, rows, dev, dense=Ai[rows].astype(np.float64)).set_b(Bx)
+        one = tl(lambda: bws.spmm_bitplanes_i8(P, K, Bx, prec, out=C))
+        e1 = ref.err(C)
+        eight = tl(lambda: bws.spmm_bitplanes_i8(P, K, Bx, prec, out=C, one_pass=False))
+        e8 = ref.err(C)
+        out[prec] = {"one_pass_ms": one, "eight_passes_ms": eight,
+                     "speedup": round(eight / one, 2),
+                     "one_pass_err_vs_fp64": e1, "eight_passes_err_vs_fp64": e8}
     return out
 
 
