@@ -223,8 +223,8 @@ def main():
     if path == "fp32_rowclusters":  # + slot rows and chunk bases
         alg_bytes += 4 * M + 4 * 2 * ((M + 15) // 16) * ((K + 31) // 32)
     achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
-    kname = {"fp32_rowclusters": "k_spmm_stream<64, 2, true>",
-             "fp32": "k_spmm_stream<64, 2, false>"}.get(path)
+    kname = {"fp32_rowclusters": "k_spmm_stream<true>",
+             "fp32": "k_spmm_stream<false>"}.get(path)
     traffic = None
     tfile = PROFILE_TRAFFIC.format(path)
     if (os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125

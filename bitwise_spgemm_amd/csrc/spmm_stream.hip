@@ -8,27 +8,33 @@
 // Workgroup: 16 waves, 256 rows x 256 columns of C; a wave owns 16 rows (its
 // "chunk"), a lane 4 columns of each (16 float4 accumulators).
 //
-// B: K is walked in stages of KS rows.  A stage's KS x 256 panel of B goes
+// B: K is walked in stages of 64 rows.  A stage's 64 x 256 panel of B goes
 // HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wave instruction, no
-// registers) into a ring of NB buffers, NB - 1 stages ahead of the stage being
-// summed.  Before a stage each wave waits only for its own copies of that
-// stage (s_waitcnt vmcnt with the younger stages' copies still in flight), then
-// one s_barrier makes every wave's copies visible and frees the buffer the
-// next copy overwrites.
+// registers) into one of two ring slots while the other is summed; one
+// s_barrier per stage makes every wave's copies visible and frees the slot
+// the next copy overwrites.  Each slot starts with a row of zeros.
 //
 // Masks: the bitmap is read as a mask stream — per chunk and stage, the
-// chunk's 16 row masks are contiguous (16 words, or 16 pairs for KS = 64) and
-// bit reversed (bit j of a stage mask <-> K row KS*stage + j).  A wave's masks
-// of a stage are one or two scalar loads (s_load_dwordx16) into SGPRs, loaded
-// one stage ahead; set bits are walked with s_ff1 / s_bitset0, so per set bit
-// the vector unit issues one LDS address and four adds, nothing else.
+// chunk's 16 row masks are contiguous 64-bit pairs, bit reversed (bit j of a
+// stage mask <-> K row 64*stage + j).  A wave's masks of the next stage are
+// loaded into SGPRs (s_load) at the end of a stage, so the wait for them falls
+// into the barrier.  Set bits are taken with s_ff1 / s_bitset0.
+//
+// Per stage a wave first takes the lowest set bit of every row without a
+// branch: s_ff1 of an empty mask is -1, which addresses the slot's zero row,
+// so the 16 reads (four in flight) and their adds run straight through.  The
+// bits left over (rows with two or more in the stage, about a third of the
+// rows at 5% density) are then walked row by row.
 //
 // Plain product: bws_mask_stream writes the stream from the bitmap (a pre-pass
 // per launch, 4 B read and 4 B written per word).  Row-clustered product: the
 // stream of the row-clustering plan (bws_cluster_rows) holds, per chunk and
 // stage, the slots' add masks (row & ~base for a delta row, the row
-// otherwise), the chunk's two base masks, a flag word (bit j: slot j has bits
-// to subtract in this stage) and the slots' subtract masks (base & ~row).
+// otherwise), the chunk's two base masks (walked as two more rows into their
+// own accumulators, added to their rows at the store), a flag word (bit j:
+// slot j has bits to subtract in this stage), the subtract masks (base & ~row)
+// of the first two flagged slots (loaded with the stage's masks) and every
+// slot's subtract mask (read on the rare stages with more than two).
 #include "mfma_common.hpp"
 
 namespace bws {
@@ -45,11 +51,15 @@ struct Rec {
   static constexpr int words = CL ? 48 * MW : 16 * MW;
   static constexpr int base = 16 * MW;      // CL: base 0, base 1
   static constexpr int flags = 18 * MW;     // CL: bit j = slot j subtracts
-  static constexpr int sub = 32 * MW;       // CL: subtract masks
+  static constexpr int inl = 19 * MW;       // CL: subtract masks of the first two flagged slots
+  static constexpr int sub = 32 * MW;       // CL: subtract masks of every slot
 };
 
 typedef const __attribute__((address_space(4))) uint32_t* sgpr_src;
 
+// A stage mask of MW words in SGPRs.  take(m): index of the lowest set bit,
+// which is cleared; -1 for an empty mask (s_ff1 of zero), which s_bitset0
+// leaves empty (it clears bit 31 or 63).  count(m): set bits.
 template <int MW>
 struct Mask;
 template <>
@@ -142,35 +152,58 @@ __device__ __forceinline__ void walk(float4& acc, typename Mask<MW>::T m, uint32
 
 __device__ __forceinline__ void s_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-// s_waitcnt vmcnt(n * RPS) for a runtime n in [0, D)
-template <int RPS, int D>
-__device__ __forceinline__ void wait_younger(int n) {
-  if (D > 3 && n >= 3) {
-    wait_vmcnt<(D > 3 ? 3 : 0) * RPS>();
-  } else if (D > 2 && n >= 2) {
-    wait_vmcnt<(D > 2 ? 2 : 0) * RPS>();
-  } else if (D > 1 && n >= 1) {
-    wait_vmcnt<RPS>();
-  } else {
-    wait_vmcnt<0>();
-  }
+// Keeps a at this point of the program (its adds are not moved into later
+// branches, where every path would need its own copy of the registers).
+__device__ __forceinline__ void pin(float4& a) {
+  asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w));
 }
 
-template <int KS, int NB, bool CL>
+// acc[r] += the LDS row of the lowest set bit of m[r], that bit cleared, for
+// every r < NR without a branch (an empty mask reads the zero row at k = -1).
+// Four rows' reads are in flight while the previous four rows are added.
+template <int MW, int NR>
+__device__ __forceinline__ void first_bits(float4 (&acc)[NR], typename Mask<MW>::T (&m)[NR],
+                                           uint32_t vbase) {
+  constexpr int G = (NR + 3) / 4;
+  f32x4 b[4], c[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < NR) b[i] = lds_row(vbase, Mask<MW>::take(m[i]));
+#pragma unroll
+  for (int g = 1; g < G; ++g) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (4 * g + i < NR) c[i] = lds_row(vbase, Mask<MW>::take(m[4 * g + i]));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      acc_add<false>(acc[4 * (g - 1) + i], b[i]);
+      pin(acc[4 * (g - 1) + i]);
+      b[i] = c[i];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (4 * (G - 1) + i < NR) {
+      acc_add<false>(acc[4 * (G - 1) + i], b[i]);
+      pin(acc[4 * (G - 1) + i]);
+    }
+}
+
+template <bool CL>
 __global__ void __launch_bounds__(kQW * 64)
 k_spmm_stream(const uint32_t* __restrict__ S, const int32_t* __restrict__ slot_row, int64_t M,
               int64_t nst, const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
               float* __restrict__ C, int64_t ldc, float alpha, float beta, int64_t row_blocks,
               int64_t col_blocks, int64_t per_xcd) {
-  static_assert(KS == 32 || KS == 64, "a stage is one or two mask words");
-  static_assert(NB >= 2 && NB <= 5 && NB * KS <= 160, "the ring must fit 160 KiB of LDS");
-  constexpr int MW = KS / 32;
-  constexpr int RPS = KS / kQW;  // copies per wave per stage (one B row each)
-  constexpr int D = NB - 1;      // stages in flight ahead of the one summed
+  constexpr int KS = 64, MW = 2;
+  constexpr int RPS = KS / kQW;               // copies per wave per stage (one B row each)
+  constexpr uint32_t SLOT = (KS + 1) * 1024;  // bytes per ring slot: the zero row, then KS rows
+  constexpr int NR = CL ? kQR + 2 : kQR;      // CL: the two bases are rows 16 and 17
   using R = Rec<CL, MW>;
   using MT = typename Mask<MW>::T;
-  constexpr bool TWO = MW == 1;
-  __shared__ float4 sB[NB * KS][64];
+  __shared__ float4 sB[2][KS + 1][64];
 
   const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
   if (tile >= row_blocks * col_blocks) return;
@@ -181,15 +214,17 @@ k_spmm_stream(const uint32_t* __restrict__ S, const int32_t* __restrict__ slot_r
   const int64_t chunk = rb * kQW + wave;
   const int64_t ncol = (n0 + 4 * lane + 4 <= N) ? n0 + 4 * lane : N - 4;  // clamped column
   const float* bcol = B + ncol;
-  const uint32_t lds0 = lds_u32(&sB[0][0]);
+  const uint32_t lds0 = lds_u32(&sB[0][0][0]);
   const sgpr_src rec = (sgpr_src)(S + chunk * nst * R::words);
+  if (wave < 2) sB[wave][0][lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  __syncthreads();
 
   // copy stage st's B rows of this wave into ring slot `slot`: B rows
   // st*KS + wave + 16j, j < RPS (rows past K read row K-1, never summed)
   const float* bw = bcol + (int64_t)wave * ldb;
   const int64_t row_step = (int64_t)kQW * ldb;  // elements between a wave's rows
   auto issue = [&](int64_t st, int slot) {
-    const uint32_t m0 = lds0 + (uint32_t)(slot * KS + wave) * 1024u;
+    const uint32_t m0 = lds0 + (uint32_t)slot * SLOT + 1024u + (uint32_t)wave * 1024u;
     if ((st + 1) * KS <= K) {
       const float* p = bw + st * KS * ldb;
 #pragma unroll
@@ -204,102 +239,57 @@ k_spmm_stream(const uint32_t* __restrict__ S, const int32_t* __restrict__ slot_r
     }
   };
 
-  float4 acc[kQR];
+  float4 acc[NR];
 #pragma unroll
-  for (int r = 0; r < kQR; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 accT0 = make_float4(0.f, 0.f, 0.f, 0.f), accT1 = accT0;
+  for (int r = 0; r < NR; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-#pragma unroll
-  for (int j = 0; j < D; ++j)
-    if (j < nst) issue(j, j);
-
-  // Mask registers.  A scalar load still in flight makes every LDS wait wait
-  // for it too (scalar loads return out of order), so a wave's scalar loads are
-  // issued once per stage, half way through its rows, when the waves of a SIMD
-  // are no longer in step.
-  //  TWO (KS = 32): two sets; the loads fill the other set with stage st + 1.
-  //  one set (KS = 64, 32 SGPRs of masks): rows 0..7 of stage st were loaded
-  //  half way through stage st - 1; half way through stage st come rows 8..15
-  //  (and the chunk words) of stage st and rows 0..7 of stage st + 1.
-  struct Set {
-    MT m[kQR];
-    MT b0, b1;
-    uint32_t fl;
-  };
-  auto load_rows = [&](Set& x, int64_t st, int r0, int r1) {
+  MT m[NR];
+  MT s0 = 0, s1 = 0;  // CL: subtract masks of the first two flagged slots
+  uint32_t fl = 0;    // CL: bit j = slot j subtracts in this stage
+  auto load = [&](int64_t st) {
     const sgpr_src p = rec + st * R::words;
 #pragma unroll
-    for (int r = r0; r < r1; ++r) x.m[r] = Mask<MW>::load(p, r);
-  };
-  auto load_chunk = [&](Set& x, int64_t st) {
+    for (int r = 0; r < kQR; ++r) m[r] = Mask<MW>::load(p, r);
     if (CL) {
-      const sgpr_src p = rec + st * R::words;
-      x.b0 = Mask<MW>::load(p + R::base, 0);
-      x.b1 = Mask<MW>::load(p + R::base, 1);
-      x.fl = p[R::flags];
+      m[NR - 2] = Mask<MW>::load(p + R::base, 0);
+      m[NR - 1] = Mask<MW>::load(p + R::base, 1);
+      fl = p[R::flags];
+      s0 = Mask<MW>::load(p + R::inl, 0);
+      s1 = Mask<MW>::load(p + R::inl, 1);
     }
   };
-  Set sa, sb;
-  load_rows(sa, 0, 0, TWO ? kQR : kQR / 2);
-  if (TWO) load_chunk(sa, 0);
-
+  issue(0, 0);
+  load(0);
   int slot = 0;
-  auto stage = [&](int64_t st, Set& cur, Set& nxt) {
-    // this wave's copies of stage st have landed (younger stages may be in flight)
-    const int64_t younger = nst - 1 - st;
-    wait_younger<RPS, D>(younger < D - 1 ? (int)younger : D - 1);
-    s_barrier();  // every wave's copies of st landed; the slot of st - 1 is free
-    {
-      int ns = slot + D;
-      ns -= ns >= NB ? NB : 0;
-      if (st + D < nst) issue(st + D, ns);
-    }
-    const uint32_t vbase = lds0 + (uint32_t)slot * (KS * 1024u) + (uint32_t)lane * 16u;
+  for (int64_t st = 0; st < nst; ++st) {
+    wait_vmcnt<0>();  // this wave's copies of stage st have landed
+    s_barrier();      // every wave's have; every wave is done with the other slot
+    if (st + 1 < nst) issue(st + 1, slot ^ 1);
+    const uint32_t vbase = lds0 + (uint32_t)slot * SLOT + 1024u + (uint32_t)lane * 16u;
+    first_bits<MW, NR>(acc, m, vbase);
 #pragma unroll
-    for (int r = 0; r < kQR; ++r) {
-      if (r == kQR / 2) {
-        if (TWO) {
-          if (st + 1 < nst) {
-            load_rows(nxt, st + 1, 0, kQR);
-            load_chunk(nxt, st + 1);
-          }
-        } else {
-          load_rows(cur, st, kQR / 2, kQR);
-          load_chunk(cur, st);
-          if (st + 1 < nst) load_rows(cur, st + 1, 0, kQR / 2);
+    for (int r = 0; r < NR; ++r) walk<MW, false>(acc[r], m[r], vbase);
+    if (CL && fl) {
+      const sgpr_src p = rec + st * R::words + R::sub;
+      int i = 0;
+#pragma unroll
+      for (int r = 0; r < kQR; ++r)
+        if ((fl >> r) & 1u) {
+          MT sm = i == 0 ? s0 : s1;
+          if (i >= 2) sm = Mask<MW>::load(p, r);
+          walk<MW, true>(acc[r], sm, vbase);
+          ++i;
         }
-      }
-      walk<MW, false>(acc[r], cur.m[r], vbase);
     }
-    if (CL) {
-      if (cur.b0) walk<MW, false>(accT0, cur.b0, vbase);
-      if (cur.b1) walk<MW, false>(accT1, cur.b1, vbase);
-      if (cur.fl) {
-        const sgpr_src p = rec + st * R::words + R::sub;
-#pragma unroll
-        for (int r = 0; r < kQR; ++r)
-          if ((cur.fl >> r) & 1u) {
-            MT sm = Mask<MW>::load(p, r);
-            walk<MW, true>(acc[r], sm, vbase);
-          }
-      }
-    }
-    slot = slot + 1 == NB ? 0 : slot + 1;
-  };
-  if (TWO) {
-    for (int64_t st = 0; st < nst; st += 2) {
-      stage(st, sa, sb);
-      if (st + 1 < nst) stage(st + 1, sb, sa);
-    }
-  } else {
-    for (int64_t st = 0; st < nst; ++st) stage(st, sa, sa);
+    if (st + 1 < nst) load(st + 1);
+    slot ^= 1;
   }
 
   // slot rows (CL) and which slots add a base, taken before any lane leaves
   int32_t srow = -1;
   if (CL) {
-    const int64_t s0 = chunk * kQR;
-    if (lane < kQR && s0 + lane < M) srow = slot_row[s0 + lane];
+    const int64_t s0r = chunk * kQR;
+    if (lane < kQR && s0r + lane < M) srow = slot_row[s0r + lane];
   }
   const bool d = lane < kQR && srow >= 0 && (srow & BWS_SLOT_DELTA);
   const uint32_t use0 = CL ? (uint32_t)__ballot(d && !(srow & BWS_SLOT_BASE1)) : 0u;
@@ -314,8 +304,8 @@ k_spmm_stream(const uint32_t* __restrict__ S, const int32_t* __restrict__ slot_r
       const int32_t sr = __builtin_amdgcn_readlane(srow, r);
       if (sr < 0) continue;
       row = sr & (BWS_SLOT_BASE1 - 1);
-      if ((use0 >> r) & 1u) acc_add<false>(v, accT0);
-      if ((use1 >> r) & 1u) acc_add<false>(v, accT1);
+      if ((use0 >> r) & 1u) acc_add<false>(v, acc[NR - 2]);
+      if ((use1 >> r) & 1u) acc_add<false>(v, acc[NR - 1]);
     } else {
       row = chunk * kQR + r;
       if (row >= M) break;
@@ -494,9 +484,10 @@ k_mask_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
 
 // Mask stream of a row-clustering plan: 16 threads per (chunk, stage)
 // record, thread j for slot j (its add and subtract masks); the record's
-// bases, flag word (bit j: slot j subtracts in this stage) and zero words are
-// written by its threads together, the chunk's base use and the flags
-// gathered with wave ballots (four records per wave).
+// bases, flag word (bit j: slot j subtracts in this stage), the first two
+// flagged slots' subtract masks and zero words are written by its threads
+// together, the chunk's base use and the flags gathered with wave ballots
+// (four records per wave).
 template <int MW>
 __global__ void __launch_bounds__(256)
 k_cluster_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
@@ -519,6 +510,7 @@ k_cluster_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   const bool used1 = ((__ballot(delta && one) >> sh) & 0xFFFFull) != 0;
   uint32_t* out = S + rec * R::words;
   uint32_t any_sub = 0;
+  uint32_t subs[MW];
 #pragma unroll
   for (int h = 0; h < MW; ++h) {
     const int64_t w = st * MW + h;
@@ -545,12 +537,24 @@ k_cluster_stream(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
         out[R::base + MW + h] = used1 ? b1 : 0u;
       }
     }
+    subs[h] = sub;
     any_sub |= sub;
   }
   const uint32_t flags = (uint32_t)((__ballot(any_sub != 0u) >> sh) & 0xFFFFull);
   if (live) {
     if (j == 0) out[R::flags] = flags;
-    for (int i = R::flags + 1 + j; i < R::sub; i += 16) out[i] = 0u;
+    // the first two flagged slots' subtract masks again, in slot order, after the flags
+    const int rank = __popc(flags & ((1u << j) - 1u));
+    if (any_sub != 0u && rank < 2) {
+#pragma unroll
+      for (int h = 0; h < MW; ++h) out[R::inl + rank * MW + h] = subs[h];
+    }
+    if (j == 0)
+      for (int q = __popc(flags); q < 2; ++q)
+#pragma unroll
+        for (int h = 0; h < MW; ++h) out[R::inl + q * MW + h] = 0u;
+    for (int i = R::flags + 1 + j; i < R::sub; i += 16)
+      if (i < R::inl || i >= R::inl + 2 * MW) out[i] = 0u;
   }
 }
 
@@ -587,26 +591,25 @@ bws_status launch_cluster_stream_t(const bws_bitmap* A, const int32_t* slot_row,
   return launch_check();
 }
 
-// The streamed FP32 product over a stream of launch_mask_stream_t (slot_row
-// NULL) or launch_cluster_stream_t (slot_row = the plan's).  N, ldb and ldc
-// multiples of 4 and B, C 16-byte aligned (checked by the callers).
-template <int KS, int NB>
+// The streamed FP32 product over a stream of launch_mask_stream_t<64> (slot_row
+// NULL) or launch_cluster_stream_t<64> (slot_row = the plan's).  N, ldb and
+// ldc multiples of 4 and B, C 16-byte aligned (checked by the callers).
 bws_status launch_spmm_stream_t(const uint32_t* S, const int32_t* slot_row, int64_t M, int64_t K,
                                 const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
                                 float alpha, float beta, hipStream_t s) {
-  const int64_t nst = ceil_div(K, KS);
+  const int64_t nst = ceil_div(K, 64);
   const int64_t row_blocks = ceil_div(M, kQRows);
   const int64_t col_blocks = ceil_div(N, kQBN);
   const int64_t per_xcd = ceil_div(row_blocks * col_blocks, 8);
   if (8 * per_xcd > kMaxGrid) return BWS_ECAPACITY;
   if (slot_row)
-    hipLaunchKernelGGL((k_spmm_stream<KS, NB, true>), dim3((unsigned)(8 * per_xcd)),
-                       dim3(kQW * 64), 0, s, S, slot_row, M, nst, B, ldb, K, N, C, ldc, alpha,
-                       beta, row_blocks, col_blocks, per_xcd);
+    hipLaunchKernelGGL(k_spmm_stream<true>, dim3((unsigned)(8 * per_xcd)), dim3(kQW * 64), 0, s,
+                       S, slot_row, M, nst, B, ldb, K, N, C, ldc, alpha, beta, row_blocks,
+                       col_blocks, per_xcd);
   else
-    hipLaunchKernelGGL((k_spmm_stream<KS, NB, false>), dim3((unsigned)(8 * per_xcd)),
-                       dim3(kQW * 64), 0, s, S, slot_row, M, nst, B, ldb, K, N, C, ldc, alpha,
-                       beta, row_blocks, col_blocks, per_xcd);
+    hipLaunchKernelGGL(k_spmm_stream<false>, dim3((unsigned)(8 * per_xcd)), dim3(kQW * 64), 0, s,
+                       S, slot_row, M, nst, B, ldb, K, N, C, ldc, alpha, beta, row_blocks,
+                       col_blocks, per_xcd);
   return launch_check();
 }
 
@@ -627,9 +630,8 @@ bws_status launch_spmm_subset_t(const uint32_t* S, int64_t M, int64_t K, const f
 }  // namespace
 
 #ifndef BWS_STREAM_NO_PRODUCT
-// The configuration the library runs: K rows per stage and LDS ring buffers.
+// K rows per stage of the streamed product.
 constexpr int kStreamKS = 64;
-constexpr int kStreamNB = 2;
 
 size_t mask_stream_bytes(int64_t M, int64_t K) { return mask_stream_bytes_t<kStreamKS>(M, K); }
 size_t cluster_stream_bytes(int64_t M, int64_t K) {
@@ -645,8 +647,7 @@ bws_status launch_cluster_stream(const bws_bitmap* A, const int32_t* slot_row,
 bws_status launch_spmm_stream(const uint32_t* S, const int32_t* slot_row, int64_t M, int64_t K,
                               const float* B, int64_t ldb, int64_t N, float* C, int64_t ldc,
                               float alpha, float beta, hipStream_t s) {
-  return launch_spmm_stream_t<kStreamKS, kStreamNB>(S, slot_row, M, K, B, ldb, N, C, ldc, alpha,
-                                                    beta, s);
+  return launch_spmm_stream_t(S, slot_row, M, K, B, ldb, N, C, ldc, alpha, beta, s);
 }
 size_t subset_stream_bytes(int64_t M, int64_t K) { return mask_stream_bytes_t<32>(M, K); }
 bws_status launch_spmm_subset(const bws_bitmap* A, uint32_t* S, const float* B, int64_t ldb,
