@@ -20,8 +20,9 @@
 //  - D: lane l holds column l % 16 and rows 4*(l/16) .. +3.
 // With the stage mask bit-reversed (bit k = K row k), lane group q's groups
 // are the nibbles of bytes 4*(q%2) + q/2 and that + 2.  A 256-entry LDS
-// table turns a mask byte into the kept values of its two nibbles, their
-// positions and the overflow byte.
+// table of 32-bit words turns a mask byte into the positions of its two
+// nibbles' kept values, whether each kept value is 1 and the overflow bits;
+// the 16-bit operand values are built from those bits in registers.
 //
 // Tiles and B staging are those of the direct-to-LDS dense kernel
 // (mfma4_tile.hpp): 4 waves, wave tile WM x WN, 64 K rows per stage copied
@@ -53,28 +54,41 @@ __device__ __forceinline__ f32x4 smfmac(const uint4& a, const U8x2& b, const f32
   }
 }
 
-// Table entry of a mask byte x (bit k = K column k of its 8): .x, .y = the
-// kept values of the low and high nibble (16-bit 1 or 0, the first kept value
-// in the low half), .z bits 0-7 = their four positions (2 bits each, low
-// nibble first), bits 8-15 = the bits of x not kept (at most 2 per nibble).
-// Per nibble v the first kept value is 1 at the position of v's lowest bit (0
-// at position 0 when v is empty), the second 1 at the position of v's next
-// bit (0, at the position after the first, when v has one bit).
-template <int PREC>
-__device__ __forceinline__ uint4 lut_entry(uint32_t x) {
-  constexpr uint32_t ONE = PREC == 2 ? 0x3C00u : 0x3F80u;
-  uint32_t val[2], pos = 0, ovf = 0;
+// Table entry of a mask byte x (bit k = K column k of its 8): bits 0-7 = the
+// four positions of the kept values (2 bits each, low nibble first), bits
+// 8-15 = the bits of x not kept (at most 2 per nibble), bits 16-19 = whether
+// the kept values are 1 (low nibble's first, second, high nibble's first,
+// second).  Per nibble v the first kept value is 1 at the position of v's
+// lowest bit (0 at position 0 when v is empty), the second 1 at the position
+// of v's next bit (0, at the position after the first, when v has one bit).
+__device__ __forceinline__ uint32_t lut_entry(uint32_t x) {
+  uint32_t pos = 0, ovf = 0, one = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const uint32_t v = (x >> (4 * h)) & 15u;
     const uint32_t r = v & (v - 1u);  // v without its lowest bit
     const uint32_t ia = v ? (uint32_t)__builtin_ctz(v) : 0u;
     const uint32_t ib = r ? (uint32_t)__builtin_ctz(r) : ((ia + 1u) & 3u);
-    val[h] = (v ? ONE : 0u) | ((r ? ONE : 0u) << 16);
     pos |= (ia | (ib << 2)) << (4 * h);
     ovf |= (r & (r - 1u)) << (4 * h);
+    one |= ((v ? 1u : 0u) | (r ? 2u : 0u)) << (2 * h);
   }
-  return make_uint4(val[0], val[1], pos | (ovf << 8), 0u);
+  return pos | (ovf << 8) | (one << 16);
+}
+
+// The kept values of the two nibbles of a table entry e as two 32-bit words
+// of 16-bit values (first kept value in the low half): with t = e's bits
+// 16-19 and z = t | (t << 15), bit 0 and bit 16 of z are the low nibble's
+// values and bits 2 and 18 the high nibble's, so (z & 0x10001) * ONE and
+// (z & 0x40004) * (ONE >> 2) place 1.0 where they are set (ONE >> 2 is exact:
+// ONE has at least 7 trailing zero bits).
+template <int PREC>
+__device__ __forceinline__ uint2 lut_values(uint32_t e) {
+  constexpr uint32_t ONE = PREC == 2 ? 0x3C00u : 0x3F80u;
+  const uint32_t t = __builtin_amdgcn_ubfe(e, 16, 4);
+  const uint32_t z = t | (t << 15);
+  return make_uint2((uint32_t)__umul24(z & 0x00010001u, ONE),
+                    (uint32_t)__umul24(z & 0x00040004u, ONE >> 2));
 }
 
 template <int WM, int WN, int NBUF>
@@ -101,7 +115,7 @@ k_spmm_smfmac(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   static_assert(NB_W >= 1 && NA_W >= 1, "stage must give every wave DMA work");
   __shared__ __attribute__((aligned(1024))) uint8_t sB[NBUF][SB];
   __shared__ __attribute__((aligned(256))) uint32_t sA[NBUF][BM * KW];
-  __shared__ __attribute__((aligned(16))) uint4 lut[256];
+  __shared__ uint32_t lut[256];
 
   const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
   if (tile >= row_blocks * col_blocks) return;
@@ -115,7 +129,7 @@ k_spmm_smfmac(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   const int64_t nst = (K + KS - 1) / KS;
   const int64_t wpr = (K + 31) / 32;
 
-  lut[tid] = lut_entry<PREC>((uint32_t)tid);  // visible after the first stage's barrier
+  lut[tid] = lut_entry((uint32_t)tid);  // visible after the first stage's barrier
 
   // per-lane DMA sources (as in the dense direct-to-LDS kernel): B row
   // r0 + t * (1024 / ROWB) of the stage, column bcol[t] (swizzled per row)
@@ -211,11 +225,12 @@ k_spmm_smfmac(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
     uint32_t ov[TM];
 #pragma unroll
     for (int a = 0; a < TM; ++a) {
-      const uint4 e0 = lut[__builtin_amdgcn_ubfe(wr[a], bsh, 8)];
-      const uint4 e2 = lut[__builtin_amdgcn_ubfe(wr[a], bsh + 16, 8)];
-      af[a] = make_uint4(e0.x, e0.y, e2.x, e2.y);
-      ia[a] = (int)((e0.z & 0xFFu) | ((e2.z & 0xFFu) << 8));
-      ov[a] = ((e0.z >> 8) & 0xFFu) | (e2.z & 0xFF00u);  // overflow bytes: x0's low, x2's high
+      const uint32_t e0 = lut[__builtin_amdgcn_ubfe(wr[a], bsh, 8)];
+      const uint32_t e2 = lut[__builtin_amdgcn_ubfe(wr[a], bsh + 16, 8)];
+      const uint2 v0 = lut_values<PREC>(e0), v2 = lut_values<PREC>(e2);
+      af[a] = make_uint4(v0.x, v0.y, v2.x, v2.y);
+      ia[a] = (int)((e0 & 0xFFu) | ((e2 & 0xFFu) << 8));
+      ov[a] = ((e0 >> 8) & 0xFFu) | (e2 & 0xFF00u);  // overflow bytes: x0's low, x2's high
     }
     __builtin_amdgcn_s_setprio(1);
     const uint32_t sbase = sB0 + (uint32_t)(buf * SB) + rd_lane;
@@ -238,9 +253,10 @@ k_spmm_smfmac(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
         for (int b = 0; b < TH; ++b)
           acc[a][b0 + b] = smfmac<PREC>(af[a], bf[b], acc[a][b0 + b], ia[a]);
         if (__ballot(ov[a] != 0u) != 0ull) {  // nibbles with 3 or 4 bits in this row tile
-          const uint4 f0 = lut[ov[a] & 0xFFu], f2 = lut[ov[a] >> 8];
-          const uint4 ag = make_uint4(f0.x, f0.y, f2.x, f2.y);
-          const int ig = (int)((f0.z & 0xFFu) | ((f2.z & 0xFFu) << 8));
+          const uint32_t f0 = lut[ov[a] & 0xFFu], f2 = lut[ov[a] >> 8];
+          const uint2 g0 = lut_values<PREC>(f0), g2 = lut_values<PREC>(f2);
+          const uint4 ag = make_uint4(g0.x, g0.y, g2.x, g2.y);
+          const int ig = (int)((f0 & 0xFFu) | ((f2 & 0xFFu) << 8));
 #pragma unroll
           for (int b = 0; b < TH; ++b)
             acc[a][b0 + b] = smfmac<PREC>(ag, bf[b], acc[a][b0 + b], ig);
