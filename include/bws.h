@@ -425,9 +425,10 @@ bws_status bws_spmm_groups(const bws_bitmap* A, const int32_t* group_id,
  * chunk of 16 slots and every 64-row stage of K, the slots' add masks
  * (row & ~base for a delta row, the row itself otherwise), the chunk's two
  * base masks (zero when no slot uses them), a flag word (bit j: slot j has
- * bits to subtract in this stage) and the slots' subtract masks (base & ~row),
- * all bit reversed (bit i <-> K row 64*stage + i) and laid out so that a
- * wave's masks of a stage are contiguous.  bws_cluster_stream_size gives the
+ * bits to subtract in this stage), the subtract masks (base & ~row) of the
+ * first two flagged slots and every slot's subtract mask, all bit reversed
+ * (bit i <-> K row 64*stage + i) and laid out so that a wave's masks of a
+ * stage are contiguous.  bws_cluster_stream_size gives the
  * bytes for a rows x cols bitmap (chunks padded to whole 256-row blocks).
  * Capturable (device only). */
 size_t bws_cluster_stream_size(int64_t rows, int64_t cols);
