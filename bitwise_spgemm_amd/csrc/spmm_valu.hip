@@ -250,14 +250,26 @@ __device__ __forceinline__ void take_bit(uint64_t& m, int& k) {
 
 // One output tile ; the
 // tile index is column-block-major within the problem.
-template <int WAVES, int RPW, int KS, bool VALS>
+// The grouped form's value tables (bws_group_rows): the value of set bit i of
+// word w of row r is gv[(G * t + g) * 32 + i] with g = gid[r * ldw + w] >= 0
+// and t = (r / T) * wpr + w; a spilled word (g < 0) takes the row's own
+// value (values in row order, or 1.0).
+struct GroupTab {
+  const int32_t* gid;
+  const float* gv;
+  int T, G;
+};
+
+template <int WAVES, int RPW, int KS, bool VALS, bool GRP = false>
 __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K,
                                            int64_t ldw, const int64_t* __restrict__ row_ptr,
                                            const float* __restrict__ values,
                                            const float* __restrict__ B, int64_t ldb, int64_t N,
                                            float* __restrict__ C, int64_t ldc, float alpha,
-                                           float beta, int64_t tile, int64_t row_blocks) {
+                                           float beta, int64_t tile, int64_t row_blocks,
+                                           GroupTab gt = GroupTab{nullptr, nullptr, 1, 1}) {
   static_assert(KS == 32 || KS == 64, "stage is one or two bitmap words");
+  static_assert(VALS || !GRP, "grouped values walk like values");
   constexpr int WPS = KS / 32;  // bitmap words per stage
   __shared__ float4 sB0[KS][64];
   __shared__ float4 sB1[KS][64];
@@ -308,18 +320,61 @@ __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64
   int64_t e_l = 0;         // VALS: lane l < RPW: value offset of row l at the current stage
   int cnt_cur = 0;         // VALS: set bits of row l in the current stage
   float vv[VALS ? RPW : 1];
-  if (VALS) e_l = row_lane ? row_ptr[mw + rl] : 0;
+  if (VALS && (!GRP || values)) e_l = row_lane ? row_ptr[mw + rl] : 0;
+  // GRP: lane p's value is that of stage column p of row r (zero when the bit
+  // is clear), so the walk reads it by column, not by rank; tbase[r] is the
+  // group-table tile index of row r's word 0
+  int64_t tbase[GRP ? RPW : 1];
+#pragma unroll
+  for (int r = 0; r < (GRP ? RPW : 1); ++r) {
+    const int64_t row = mw + r < M ? mw + r : M - 1;
+    tbase[r] = GRP ? (row / gt.T) * wpr : 0;
+  }
+  auto grp_values = [&](float (&dst)[VALS ? RPW : 1], uint32_t words, int64_t st, int64_t e_row) {
+    const uint64_t sm = stage_mask(st);
+    const int p = lane;
+    const int64_t wp = WPS * st + (p >> 5);
+    const int64_t wc = wp < wpr ? wp : wpr - 1;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const uint32_t whi = (uint32_t)__builtin_amdgcn_readlane((int)words, r);
+      const uint32_t wlo = WPS == 2 ? (uint32_t)__builtin_amdgcn_readlane((int)words, r + 32) : 0u;
+      const uint64_t m = ((((uint64_t)whi) << 32) | wlo) & sm;
+      const bool set = p < KS && st < nst && ((m >> (63 - p)) & 1ull) != 0;
+      int64_t row = mw + r;
+      row = row < M ? row : M - 1;
+      const int32_t g = gt.gid[row * ldw + wc];  // clamped, unconditional
+      const int64_t t = tbase[GRP ? r : 0] + wp;
+      const int64_t er = readlane_i64(e_row, r);
+      float v = 0.0f;
+      if (set) {
+        if (g >= 0) {
+          v = gt.gv[(t * gt.G + g) * 32 + (p & 31)];
+        } else if (values) {
+          const int rank = p == 0 ? 0 : __popcll(m >> (64 - p));
+          v = values[er + rank];
+        } else {
+          v = 1.0f;
+        }
+      }
+      dst[VALS ? r : 0] = v;
+    }
+  };
 
   issue(0, base0);
   uint32_t wcur = load_words(0);
   uint32_t wnext = load_words(1);
   if (VALS) {
     cnt_cur = __popcll(lane_row_mask(wcur, stage_mask(0)));
+    if (GRP) {
+      grp_values(vv, wcur, 0, e_l);
+    } else {
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      const int64_t e = readlane_i64(e_l, r);
-      const int c = __builtin_amdgcn_readlane(cnt_cur, r);
-      vv[r] = lane < c ? values[e + lane] : 0.0f;
+      for (int r = 0; r < RPW; ++r) {
+        const int64_t e = readlane_i64(e_l, r);
+        const int c = __builtin_amdgcn_readlane(cnt_cur, r);
+        vv[r] = lane < c ? values[e + lane] : 0.0f;
+      }
     }
   }
 
@@ -343,11 +398,15 @@ __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64
     if (VALS) {
       const int64_t e_next = e_l + cnt_cur;
       cnt_next = more ? __popcll(lane_row_mask(wnext, stage_mask(st + 1))) : 0;
+      if (GRP) {
+        grp_values(vn, wnext, st + 1, e_next);
+      } else {
 #pragma unroll
-      for (int r = 0; r < RPW; ++r) {
-        const int64_t e = readlane_i64(e_next, r);
-        const int c = __builtin_amdgcn_readlane(cnt_next, r);
-        vn[r] = lane < c ? values[e + lane] : 0.0f;
+        for (int r = 0; r < RPW; ++r) {
+          const int64_t e = readlane_i64(e_next, r);
+          const int c = __builtin_amdgcn_readlane(cnt_next, r);
+          vn[r] = lane < c ? values[e + lane] : 0.0f;
+        }
       }
     }
 
@@ -372,10 +431,10 @@ __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64
         const float4 b0 = cur[k0][lane], b1 = cur[k1][lane], b2 = cur[k2][lane],
                      b3 = cur[k3][lane];
         if (VALS) {
-          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b0);
-          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 1), b1);
-          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 2), b2);
-          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 3), b3);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k0 : j), b0);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k1 : j + 1), b1);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k2 : j + 2), b2);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k3 : j + 3), b3);
         } else {
           add4(acc[r], b0);
           add4(acc[r], b1);
@@ -391,8 +450,8 @@ __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64
         take_bit(m, k1);
         const float4 b0 = cur[k0][lane], b1 = cur[k1][lane];
         if (VALS) {
-          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b0);
-          fma4(acc[r], readlane_f(vv[VALS ? r : 0], j + 1), b1);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k0 : j), b0);
+          fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k1 : j + 1), b1);
         } else {
           add4(acc[r], b0);
           add4(acc[r], b1);
@@ -404,7 +463,7 @@ __device__ __forceinline__ void rows2_tile(const uint32_t* __restrict__ W, int64
         int k0;
         take_bit(m, k0);
         const float4 b0 = cur[k0][lane];
-        if (VALS) fma4(acc[r], readlane_f(vv[VALS ? r : 0], j), b0);
+        if (VALS) fma4(acc[r], readlane_f(vv[VALS ? r : 0], GRP ? k0 : j), b0);
         else add4(acc[r], b0);
       }
     }
@@ -457,6 +516,20 @@ k_spmm_rows2(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
   if (tile >= row_blocks * col_blocks) return;
   rows2_tile<WAVES, RPW, KS, VALS>(W, M, K, ldw, row_ptr, values, B, ldb, N, C, ldc, alpha, beta,
                                    tile, row_blocks);
+}
+
+// Grouped-form evaluation (bws_spmm_groups) on the LDS-staged tile.
+template <int WAVES, int RPW, int KS>
+__global__ void __launch_bounds__(WAVES * 64)
+k_spmm_rows2_groups(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+                    const int64_t* __restrict__ row_ptr, const float* __restrict__ values,
+                    GroupTab gt, const float* __restrict__ B, int64_t ldb, int64_t N,
+                    float* __restrict__ C, int64_t ldc, int64_t row_blocks, int64_t col_blocks,
+                    int64_t per_xcd) {
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= row_blocks * col_blocks) return;
+  rows2_tile<WAVES, RPW, KS, true, true>(W, M, K, ldw, row_ptr, values, B, ldb, N, C, ldc, 1.0f,
+                                         0.0f, tile, row_blocks, gt);
 }
 
 // Grouped launch: workgroup g takes global tile xcd_tile(g), finds its problem
@@ -716,6 +789,21 @@ bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_
 bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
                               int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, hipStream_t s) {
+  // B rows staged in LDS by 16-byte copies when the rows allow it (the tile
+  // kernel of the value-carrying FP32 product, values from the group tables)
+  if (valu_direct_ok(BWS_DTYPE_F32, B, ldb, N, C, ldc)) {
+    constexpr int WV = 16, RP = 8, KS = 64;
+    const int64_t row_blocks = ceil_div(A->rows, WV * RP);
+    const int64_t cbs = ceil_div(N, kBN);
+    const int64_t per_xcd = ceil_div(row_blocks * cbs, 8);
+    if (8 * per_xcd <= kMaxGrid) {
+      hipLaunchKernelGGL((k_spmm_rows2_groups<WV, RP, KS>), dim3((unsigned)(8 * per_xcd)),
+                         dim3(WV * 64), 0, s, A->words, A->rows, A->cols, A->ldw, A->row_ptr,
+                         A->values, GroupTab{group_id, gv, T, G}, B, ldb, N, C, ldc, row_blocks,
+                         cbs, per_xcd);
+      return launch_check();
+    }
+  }
   const int64_t col_blocks = ceil_div(N, 256);
   hipLaunchKernelGGL(k_spmm_groups, dim3(grid_for(A->rows * col_blocks, 4)), dim3(256), 0, s,
                      A->words, A->rows, A->cols, A->ldw, A->row_ptr, A->values, group_id, gv,
