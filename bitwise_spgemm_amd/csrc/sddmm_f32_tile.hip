@@ -205,8 +205,15 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
   if (tile >= row_blocks * k_blocks) return;
-  // row blocks vary fastest: the tiles an XCD runs together share B's rows
-  const int64_t kb = tile / row_blocks, rb = tile - kb * row_blocks;
+  // groups of 8 row blocks, column blocks in order within a group, row blocks
+  // fastest: the ~64 tiles an XCD runs together are 8 row blocks x 8 column
+  // blocks, so each stage's slice of a dC row or a B row is fetched from HBM
+  // once for 8 tiles (row blocks fastest over all of them had every tile of
+  // a column block read its own dC rows: dC came from HBM 64 times)
+  const int64_t gsz = 8 * k_blocks, grp = tile / gsz, first = grp * 8;
+  const int64_t rows_g = row_blocks - first < 8 ? row_blocks - first : 8;
+  const int64_t in = tile - grp * gsz;
+  const int64_t rb = first + in % rows_g, kb = in / rows_g;
   const int64_t r0 = rb * TT, k0 = kb * TT;
   const int64_t wpr = words_for(K);
 
@@ -230,7 +237,7 @@ k_sddmm_f32_tile(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t l
   }
   __syncthreads();
   const int total = sTotal;
-  if (tid == 0) dense_tile[tile] = total >= SPARSE ? 1 : 0;
+  if (tid == 0) dense_tile[kb * row_blocks + rb] = total >= SPARSE ? 1 : 0;
   if (total < SPARSE) return;
   if (row_in && cnt > 0) sBase[lane] = row_ptr[wr] + woff[wr * wpr + w0];
   const int n_items = sItem[TT];
