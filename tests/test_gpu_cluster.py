@@ -160,3 +160,41 @@ def test_spmm_clustered_rejects(bws, dev):
     rp = torch.zeros(65, dtype=torch.int64, device=dev)
     with pytest.raises(bws.BwsError):
         bws.cluster_rows(A.with_values(torch.zeros(1, device=dev), rp))
+
+
+def test_spmm_clustered_many_subtracts(oracle, bws, dev, rng):
+    """Rows of one dense prototype (50%) with a fifth of its bits dropped and 1%
+    added per row: nearly every slot subtracts in nearly every stage, so the
+    subtract masks come from the stream's inline pair and from its full table
+    (third and later flagged slots of a stage)."""
+    M, K, N = 512, 1000, 132
+    proto = rng.random(K) < 0.5
+    drop = rng.random((M, K)) < 0.2
+    add = rng.random((M, K)) < 0.01
+    D = ((proto[None, :] & ~drop) | add).astype(np.float32)
+    W = oracle.pack_bitmap(D)
+    A = bws.Bitmap(to_dev(W, dev), M, K)
+    B = bws.gen_f32(12, (K, N), "normal", 0.0, 1.0)
+    pl = bws.cluster_rows(A)
+    # the plan makes these delta rows, and some chunk-stage has three or more
+    # subtracting slots
+    slot = pl.slot_row.cpu().numpy().astype(np.int64)
+    cm = to_host_u32(pl.chunk_mask)
+    delta = (slot & bws.SLOT_DELTA) != 0
+    assert delta.mean() > 0.9
+    Wp = oracle.unpack_bitmap(W, K) != 0
+    most = 0
+    for c in range(M // 16):
+        flagged = np.zeros((K + 63) // 64, np.int64)
+        for j in range(16):
+            sr = slot[16 * c + j]
+            if sr < 0 or not (sr & bws.SLOT_DELTA):
+                continue
+            base = oracle.unpack_bitmap(cm[c, 1 if sr & bws.SLOT_BASE1 else 0][None, :], K)[0] != 0
+            sub = base & ~Wp[sr & (bws.SLOT_BASE1 - 1)]
+            flagged += np.add.reduceat(sub, np.arange(0, K, 64)) > 0
+        most = max(most, int(flagged.max()))
+    assert most >= 3
+    C = bws.spmm_clustered(A, pl, to_dev(B, dev)).cpu().numpy()
+    e = rel_err(oracle, C, W, K, B)
+    assert e["max_rel_s"] <= 1e-5, e
