@@ -442,6 +442,7 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
         ex["density_sweep"] = _density_sweep(bws, dev, M, N, K, args)
         ex["value_mask"] = _value_mask_workload(bws, dev)
         ex["int8_bitplanes"] = _bitplanes_workload(bws, dev, Bd, M, N, K)
+        ex["mfma_split"] = _split_workload(bws, dev, Bd, M, N, K)
         ex["grouped"] = _grouped_workload(bws, dev)
         ex["comparators"] = _comparators(bws, A, W, Bd, M, N, K, nnz, dev)
     return ex
@@ -524,6 +525,41 @@ def _bitplanes_workload(bws, dev, Bd, M, N, K):
         out[prec] = {"one_pass_ms": one, "eight_passes_ms": eight,
                      "speedup": round(eight / one, 2),
                      "one_pass_err_vs_fp64": e1, "eight_passes_err_vs_fp64": e8}
+    return out
+
+
+def _split_workload(bws, dev, Bd, M, N, K):
+    """BF16 on a 0/1 mask whose 128-row blocks alternate between 10% and
+    0.05% density (every 128 x 64 block of the sparse ones still has bits, so
+    the MFMA kernel skips none of its stages): every block on MFMA against the
+    density split (plan mfma_min_bits 40: the sparse blocks on the vector
+    kernel; mfma_min_bits 1 sends every block with a bit to the flagged-block
+    MFMA kernel); errors against FP64 (of the BF16 B) on sampled rows."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(11)
+    W = np.zeros((M, (K + 31) // 32), np.uint32)
+    for b0 in range(0, M, 128):
+        p = 0.1 if (b0 // 128) % 2 == 0 else 0.0005
+        W[b0:b0 + 128] = bws.gen_bitmap_iid(int(rng.integers(1 << 30)), min(128, M - b0), K, p)
+    A = bws.from_host_words(W, M, K, dev)
+    nnz = int(np.unpackbits(W.view(np.uint8)).sum())
+    Bb = Bd.to(torch.bfloat16)
+    rows = sample_rows(M)
+    ref = Fp64Ref(W, K, rows, dev).set_b(Bb)
+    C = torch.empty((M, N), device=dev)
+    out = {"workload": f"0/1 mask {M}x{K}, 128-row blocks alternating 10% / 0.05% "
+                       f"(nnz {nnz}) @ {K}x{N}, BF16 B"}
+    for name, plan in (("all_mfma", bws.make_plan(kernel=2)),
+                       ("split_1", bws.make_plan(mfma_min_bits=1)),
+                       ("split_40", bws.make_plan(mfma_min_bits=40))):
+        f = lambda: bws.spmm(A, Bb, "bf16", out=C, plan=plan)
+        spin_up(f, 0.05)
+        ms = time_loop(f, 10, 2)[1]
+        out[name + "_ms"] = round(ms, 4)
+        out[name + "_tflops"] = round(2 * nnz * N / (ms * 1e-3) / 1e12, 2)
+        out[name + "_err_vs_fp64"] = ref.err(C)
+    out["speedup"] = round(out["all_mfma_ms"] / out["split_40_ms"], 3)
     return out
 
 

@@ -685,7 +685,8 @@ def spmm(A: Bitmap, B: torch.Tensor, precision: str = "fp32", alpha: float = 1.0
     s = A.struct()
     pl = None if plan is None else C.byref(plan)
     wsz = lib().bws_spmm_workspace_size(C.byref(s), N, PRECISION[precision], pl)
-    if precision in ("bf16", "fp16") and A.values is None and B.dtype != torch.float32:
+    if (precision in ("bf16", "fp16") and A.values is None and B.dtype != torch.float32
+            and (plan is None or plan.mfma_min_bits == 0)):
         wsz = 0  # the workspace would hold a rounded FP32 B: not needed for 16-bit B
     ws = _workspace(wsz, dev) if wsz else None
     _check(lib().bws_spmm(C.byref(s), _ptr(B), DTYPE[B.dtype], B.stride(0), N, _ptr(out),

@@ -121,6 +121,23 @@ constexpr int32_t kSubsetKStage = 16;
 constexpr int64_t kStreamTileRows = 256;
 constexpr int64_t kStreamTileCols = 256;
 
+// MFMA density split (plan mfma_min_bits > 0, 16-bit precision, 0/1 A): the
+// rows in blocks of kSplitRows; the flags pass (per-block bit counts, a byte
+// per block: 1 = MFMA, and the list of flagged blocks, list[0] = their count)
+// in a workspace of split_ws_bytes, and the two sides of the product.
+constexpr int64_t kSplitRows = 128;
+size_t split_ws_bytes(int64_t M);
+bws_status launch_split_flags(const bws_bitmap* A, int32_t min_bits, void* ws, uint8_t** flags,
+                              int32_t** list, hipStream_t s);
+bws_status spmm_valu_split16(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb,
+                             int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                             const uint8_t* mfma_rows, hipStream_t s);
+bws_status launch_round_b16(const float* B, int64_t K, int64_t N, int64_t ldb, int32_t precision,
+                            uint16_t* Bh, hipStream_t s);
+bws_status spmm_mfma_split16(const bws_bitmap* A, const void* B, int32_t precision, int64_t ldb,
+                             int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                             const int32_t* list, hipStream_t s);
+
 // Device scratch carving: a linear allocator over the caller's workspace.
 struct Carve {
   char* base;

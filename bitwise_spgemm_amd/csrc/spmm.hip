@@ -167,6 +167,10 @@ size_t bws_spmm_workspace_size(const bws_bitmap* A, int64_t N, int32_t precision
   if (precision == BWS_PREC_FP32 && !A->values && N % 4 == 0 && (!plan || plan->rows_per_wg == 0))
     return mask_stream_bytes(A->rows, A->cols);
   if (plan && plan->kernel == 1) return 0;
+  // the density split: the row-block flags and room to round an FP32 B
+  if (!A->values && plan && plan->mfma_min_bits > 0 && N % 8 == 0 &&
+      (precision == BWS_PREC_BF16 || precision == BWS_PREC_FP16))
+    return split_ws_bytes(A->rows) + spmm_round_workspace(A->cols, N);
   if (!A->values) {
     if (N % 8 != 0) return 0;
     if (precision == BWS_PREC_BF16X2) return spmm_split_workspace(A->cols, N);
@@ -227,6 +231,36 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
       workspace_bytes >= spmm_vals_workspace(A) &&
       (reinterpret_cast<uintptr_t>(workspace) & 255u) == 0)
     return spmm_mfma_vals(A, B, precision, ldb, N, C, ldc, alpha, beta, workspace, stream);
+  // BF16 / FP16 on a 0/1 A with the plan's density split (mfma_min_bits > 0):
+  // the 128-row blocks with fewer than mfma_min_bits set bits per 64 K columns
+  // take the vector kernel, the others the direct-to-LDS MFMA kernel (MFMA
+  // only where the block is dense enough to be a real contraction); the block
+  // flags, and an FP32 B rounded once, live in the workspace.
+  if (!A->values && A->cols > 0 && (precision == BWS_PREC_BF16 || precision == BWS_PREC_FP16) &&
+      plan && plan->mfma_min_bits > 0 && plan->kernel != 1 && N % 8 == 0) {
+    const size_t fb = split_ws_bytes(A->rows);
+    const bool b16 = b_dtype != BWS_DTYPE_F32;
+    if (ws_ok(workspace, workspace_bytes, fb + (b16 ? 0 : spmm_round_workspace(A->cols, N)))) {
+      const void* Bh = B;
+      int64_t ldbh = ldb;
+      int32_t bd = b_dtype;
+      if (!b16) {
+        uint16_t* r = reinterpret_cast<uint16_t*>(static_cast<uint8_t*>(workspace) + fb);
+        BWS_TRY(launch_round_b16(static_cast<const float*>(B), A->cols, N, ldb, precision, r,
+                                 stream));
+        Bh = r;
+        ldbh = N;
+        bd = precision == BWS_PREC_FP16 ? BWS_DTYPE_F16 : BWS_DTYPE_BF16;
+      }
+      if (dma_ok(precision, bd, Bh, ldbh, N)) {
+        uint8_t* flags = nullptr;
+        int32_t* list = nullptr;
+        BWS_TRY(launch_split_flags(A, plan->mfma_min_bits, workspace, &flags, &list, stream));
+        BWS_TRY(spmm_mfma_split16(A, Bh, precision, ldbh, N, C, ldc, alpha, beta, list, stream));
+        return spmm_valu_split16(A, Bh, bd, ldbh, N, C, ldc, alpha, beta, flags, stream);
+      }
+    }
+  }
   // BF16 / FP16 with FP32 B on a 0/1 A: with room for the rounded B, round it
   // once and run the direct-to-LDS kernel; otherwise round in registers.
   if (!A->values && A->cols > 0 && (precision == BWS_PREC_BF16 || precision == BWS_PREC_FP16) &&

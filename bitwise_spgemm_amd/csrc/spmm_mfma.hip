@@ -736,6 +736,129 @@ bws_status spmm_mfma_batch(const BatchEntry* entries, const int64_t* tile_off, i
   return BWS_EUNSUPPORTED;
 }
 
+namespace {
+
+constexpr int kSplitSlices = 16;  // partial counts per block (8 rows each)
+
+// Set bits of rows [128 b + 8 s, +8) of the split's row block b, slice s.
+__global__ void __launch_bounds__(256)
+k_split_counts(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+               int32_t* __restrict__ partial) {
+  __shared__ int32_t part[4];
+  const int64_t r0 = (int64_t)blockIdx.x * kSplitRows + (int64_t)blockIdx.y * (kSplitRows / kSplitSlices);
+  int64_t rows = M - r0;
+  rows = rows < 0 ? 0 : (rows < kSplitRows / kSplitSlices ? rows : kSplitRows / kSplitSlices);
+  const int64_t wpr = words_for(K), n = rows * wpr;
+  int32_t c = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const int64_t r = i / wpr, w = i - r * wpr;
+    c += __popc(W[(r0 + r) * ldw + w] & valid_bits(w, K));
+  }
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partial[blockIdx.x * kSplitSlices + blockIdx.y] = part[0] + part[1] + part[2] + part[3];
+}
+
+// One workgroup: flags[b] = the block has at least `threshold` bits, and the
+// flagged blocks in order, list[0] = their count, list[1 ..] = the blocks.
+__global__ void __launch_bounds__(256)
+k_split_list(const int32_t* __restrict__ partial, int64_t blocks, int64_t threshold,
+             uint8_t* __restrict__ flags, int32_t* __restrict__ list) {
+  __shared__ int32_t wsum[4];
+  __shared__ int32_t base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  for (int64_t b0 = 0; b0 < blocks; b0 += 256) {
+    const int64_t b = b0 + threadIdx.x;
+    int64_t tot = 0;
+    if (b < blocks)
+      for (int s = 0; s < kSplitSlices; ++s) tot += partial[b * kSplitSlices + s];
+    const int f = (b < blocks && tot >= threshold) ? 1 : 0;
+    if (b < blocks) flags[b] = (uint8_t)f;
+    const int incl = wave_incl_scan(f);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) off += wsum[w];
+    if (f) list[1 + off + incl - 1] = (int32_t)b;
+    __syncthreads();
+    if (threadIdx.x == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) list[0] = base;
+}
+
+// The MFMA side of the split: the flagged row blocks' tiles, numbered column
+// block by column block (tile t: column block t / count, row block list[1 +
+// t % count]), handed out in contiguous ranges per XCD as the full kernel does,
+// so that one XCD's L2 holds few column blocks of B.  Workgroups beyond each
+// XCD's share (the count is only known on the device) exit at once.
+template <int PREC, int WN>
+__global__ void __launch_bounds__(256, 2)
+k_spmm_mfma4_list(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+                  const uint16_t* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
+                  int64_t ldc, float alpha, float beta, int64_t row_blocks, int64_t col_blocks,
+                  const int32_t* __restrict__ list) {
+  const int64_t nm = list[0];
+  const int64_t total = nm * col_blocks, per_xcd = (total + 7) >> 3;
+  if ((int64_t)(blockIdx.x >> 3) >= per_xcd) return;
+  const int64_t g = xcd_tile(blockIdx.x, per_xcd);
+  if (g >= total) return;
+  const int64_t rb = list[1 + g % nm], cb = g / nm;
+  mfma4_tile<PREC, 64, WN, 2, 64>(W, M, K, ldw, B, ldb, N, C, ldc, alpha, beta,
+                                  cb * row_blocks + rb, row_blocks);
+}
+
+}  // namespace
+
+size_t split_ws_bytes(int64_t M) {
+  const int64_t blocks = ceil_div(M, kSplitRows);
+  return (size_t)(ceil_div(blocks * kSplitSlices * 4, 256) + ceil_div(blocks, 256) +
+                  ceil_div((blocks + 1) * 4, 256)) * 256;
+}
+
+bws_status launch_split_flags(const bws_bitmap* A, int32_t min_bits, void* ws, uint8_t** flags,
+                              int32_t** list, hipStream_t s) {
+  const int64_t blocks = ceil_div(A->rows, kSplitRows);
+  if (blocks > 65535) return BWS_ECAPACITY;
+  Carve cv(ws);
+  int32_t* partial = cv.take<int32_t>((size_t)(blocks * kSplitSlices));
+  *flags = cv.take<uint8_t>((size_t)blocks);
+  *list = cv.take<int32_t>((size_t)(blocks + 1));
+  hipLaunchKernelGGL(k_split_counts, dim3((unsigned)blocks, kSplitSlices), dim3(256), 0, s,
+                     A->words, A->rows, A->cols, A->ldw, partial);
+  BWS_TRY(launch_check());
+  hipLaunchKernelGGL(k_split_list, dim3(1), dim3(256), 0, s, partial, blocks,
+                     (int64_t)min_bits * ceil_div(A->cols, 64), *flags, *list);
+  return launch_check();
+}
+
+// The MFMA side of the density split: the direct-to-LDS kernel with 128 x 128
+// workgroup tiles (64 x 64 per wave) on the flagged row blocks.  The tile is
+// narrower than the full kernel's 128 x 256 at large N because one tile's
+// K loop is latency-bound: with half the row blocks flagged, 128 x 256 tiles
+// would leave half of each CU's two workgroup slots empty.
+bws_status spmm_mfma_split16(const bws_bitmap* A, const void* B, int32_t precision, int64_t ldb,
+                             int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                             const int32_t* list, hipStream_t s) {
+  static_assert(2 * 64 == kSplitRows, "MFMA row blocks are the split blocks");
+  const int64_t row_blocks = ceil_div(A->rows, kSplitRows), col_blocks = ceil_div(N, 128);
+  const int64_t grid = 8 * ceil_div(row_blocks * col_blocks, 8);
+  if (grid > kMaxGrid) return BWS_ECAPACITY;
+  const uint16_t* b = static_cast<const uint16_t*>(B);
+  if (precision == BWS_PREC_FP16)
+    hipLaunchKernelGGL((k_spmm_mfma4_list<2, 64>), dim3((unsigned)grid), dim3(256), 0, s, A->words,
+                       A->rows, A->cols, A->ldw, b, ldb, N, C, ldc, alpha, beta, row_blocks,
+                       col_blocks, list);
+  else
+    hipLaunchKernelGGL((k_spmm_mfma4_list<1, 64>), dim3((unsigned)grid), dim3(256), 0, s, A->words,
+                       A->rows, A->cols, A->ldw, b, ldb, N, C, ldc, alpha, beta, row_blocks,
+                       col_blocks, list);
+  return launch_check();
+}
+
 bws_status spmm_mfma(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb, int64_t N,
                      float* C, int64_t ldc, float alpha, float beta, int32_t precision,
                      const bws_plan* plan, hipStream_t s) {
@@ -829,6 +952,23 @@ size_t spmm_round_workspace(int64_t K, int64_t N) { return split_plane_bytes(K, 
 // (row-major, ld N) and the direct-to-LDS 16-bit kernel multiplies it; the
 // rounding and the K order are those of the register-staged kernel, so the
 // result is the same bit for bit, without staging B through registers.
+// FP32 B rounded once into out (K x N, ld N) in the precision's 16-bit type.
+bws_status launch_round_b16(const float* B, int64_t K, int64_t N, int64_t ldb, int32_t precision,
+                            uint16_t* Bh, hipStream_t s) {
+  const int64_t quads = K * (N / 4);
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(quads, 256), 16384);
+  const bool vec = ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0;
+  uint2* out = reinterpret_cast<uint2*>(Bh);
+  if (precision == BWS_PREC_FP16) {
+    if (vec) hipLaunchKernelGGL((k_round_b16<2, true>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+    else hipLaunchKernelGGL((k_round_b16<2, false>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+  } else {
+    if (vec) hipLaunchKernelGGL((k_round_b16<1, true>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+    else hipLaunchKernelGGL((k_round_b16<1, false>), dim3(grid), dim3(256), 0, s, B, K, N, ldb, out);
+  }
+  return launch_check();
+}
+
 bws_status spmm_mfma_round(const bws_bitmap* A, const float* B, int64_t ldb, int64_t N, float* C,
                            int64_t ldc, float alpha, float beta, int32_t precision,
                            const bws_plan* plan, void* workspace, hipStream_t s) {

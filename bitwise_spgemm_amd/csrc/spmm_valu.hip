@@ -786,6 +786,123 @@ bws_status spmm_valu(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_
 #undef BWS_ROWS
 }
 
+namespace {
+
+// The vector side of the MFMA density split: one wave per (row, 256 columns)
+// of the row blocks flagged 0; per set bit the wave reads the 256 columns of
+// B's row straight from memory (8 bytes of 16-bit values per lane, four bits'
+// rows in flight) and adds them in FP32.  The rows these blocks hold are
+// sparse (that is why they were not given to MFMA), so staging B's panels in
+// LDS for them would cost more than the bits.
+template <int TB>
+__global__ void __launch_bounds__(256)
+k_spmm_split_rows(const uint32_t* __restrict__ W, int64_t M, int64_t K, int64_t ldw,
+                  const uint16_t* __restrict__ B, int64_t ldb, int64_t N, float* __restrict__ C,
+                  int64_t ldc, float alpha, float beta, const uint8_t* __restrict__ mfma_rows,
+                  int64_t col_blocks) {
+  const int lane = threadIdx.x & 63;
+  const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (item >= M * col_blocks) return;
+  const int64_t r = item / col_blocks, cb = item - r * col_blocks;
+  if (mfma_rows[r / kSplitRows]) return;
+  const int64_t n = cb * 256 + 4 * lane;
+  const int64_t nc = n + 4 <= N ? n : N - 4;  // lanes past N read the last columns, never store
+  const int64_t wpr = words_for(K);
+  const uint16_t* bcol = B + nc;
+  auto cvt2 = [](uint32_t u, float& lo, float& hi) {
+    if (TB == 1) {
+      lo = __uint_as_float(u << 16);
+      hi = __uint_as_float(u & 0xFFFF0000u);
+    } else {
+      _Float16 h[2];
+      __builtin_memcpy(h, &u, 4);
+      lo = (float)h[0];
+      hi = (float)h[1];
+    }
+  };
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  // the row's words 64 at a time, a word per lane; the set bits taken in
+  // column order, four at a time, across words (uniform scalar work)
+  for (int64_t w0 = 0; w0 < wpr; w0 += 64) {
+    const int64_t wl = w0 + lane;
+    const uint32_t word = wl < wpr ? W[r * ldw + wl] & valid_bits(wl, K) : 0u;
+    uint64_t nz = __ballot(word != 0u);
+    uint32_t cur = 0;
+    int64_t cw = 0;
+    while (nz != 0ull || cur != 0u) {
+      int64_t k[4];
+      int cnt = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (cur == 0u && nz != 0ull) {
+          const int l = __builtin_ctzll(nz);
+          nz &= nz - 1ull;
+          cw = w0 + l;
+          cur = (uint32_t)__builtin_amdgcn_readlane((int)word, l);
+        }
+        k[i] = 32 * cw;  // a row of B that exists (never added when i >= cnt)
+        if (cur != 0u) {
+          const int b = __clz((int)cur);
+          cur &= ~wbit(b);
+          k[i] += b;
+          ++cnt;
+        }
+      }
+      uint2 u[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) u[i] = *reinterpret_cast<const uint2*>(bcol + k[i] * ldb);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i < cnt) {  // uniform: the row's bits are the wave's
+          float x0, x1, x2, x3;
+          cvt2(u[i].x, x0, x1);
+          cvt2(u[i].y, x2, x3);
+          acc.x += x0;
+          acc.y += x1;
+          acc.z += x2;
+          acc.w += x3;
+        }
+      }
+    }
+  }
+  if (n >= N) return;
+  float4* c = reinterpret_cast<float4*>(C + r * ldc + n);
+  float4 v = make_float4(alpha * acc.x, alpha * acc.y, alpha * acc.z, alpha * acc.w);
+  if (beta != 0.0f) {
+    const float4 o = *c;
+    v.x = fmaf(beta, o.x, v.x);
+    v.y = fmaf(beta, o.y, v.y);
+    v.z = fmaf(beta, o.z, v.z);
+    v.w = fmaf(beta, o.w, v.w);
+  }
+  *c = v;
+}
+
+}  // namespace
+
+// The vector side of the MFMA density split: the 0/1 product with 16-bit B
+// (bf16 or f16; N % 8 == 0, 16-byte aligned rows of B and C) on the row
+// blocks whose flag is 0.
+bws_status spmm_valu_split16(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t ldb,
+                             int64_t N, float* C, int64_t ldc, float alpha, float beta,
+                             const uint8_t* mfma_rows, hipStream_t s) {
+  const int64_t col_blocks = ceil_div(N, 256);
+  const int64_t items = A->rows * col_blocks;
+  if (ceil_div(items, 4) > kMaxGrid) return BWS_ECAPACITY;
+  const uint16_t* b = static_cast<const uint16_t*>(B);
+  if (b_dtype == BWS_DTYPE_BF16)
+    hipLaunchKernelGGL(k_spmm_split_rows<1>, dim3((unsigned)ceil_div(items, 4)), dim3(256), 0, s,
+                       A->words, A->rows, A->cols, A->ldw, b, ldb, N, C, ldc, alpha, beta,
+                       mfma_rows, col_blocks);
+  else if (b_dtype == BWS_DTYPE_F16)
+    hipLaunchKernelGGL(k_spmm_split_rows<2>, dim3((unsigned)ceil_div(items, 4)), dim3(256), 0, s,
+                       A->words, A->rows, A->cols, A->ldw, b, ldb, N, C, ldc, alpha, beta,
+                       mfma_rows, col_blocks);
+  else
+    return BWS_EUNSUPPORTED;
+  return launch_check();
+}
+
 bws_status spmm_groups_launch(const bws_bitmap* A, const int32_t* group_id, const float* gv,
                               int32_t T, int32_t G, const float* B, int64_t ldb, int64_t N,
                               float* C, int64_t ldc, hipStream_t s) {

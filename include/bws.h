@@ -327,7 +327,13 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
  *                 rows; one read and four adds per row nibble, whatever its
  *                 bits: faster than the bit walk above about 40% density),
  *                 with the workspace bws_spmm_workspace_size reports for it
- *   mfma_min_bits reserved for the density classifier (0 = default)
+ *   mfma_min_bits BF16 / FP16 on a 0/1 A (0 = off, every row block on MFMA):
+ *                 128-row blocks with fewer than mfma_min_bits set bits per
+ *                 64 K columns (on average) run on the vector kernel, the
+ *                 others on the direct-to-LDS MFMA kernel; needs N % 8 == 0,
+ *                 16-byte aligned rows of B (or of B rounded once, for FP32
+ *                 B) and the workspace bws_spmm_workspace_size reports (the
+ *                 block flags and room to round an FP32 B); otherwise ignored
  *   cols_per_wg   MFMA path: columns of C per workgroup (0 = auto, 128 or 256;
  *                 256 x 256 only with the direct-to-LDS staging of 16-bit B,
  *                 else it runs as 128 x 256; BF16X2 uses 128)

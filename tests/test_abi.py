@@ -277,3 +277,13 @@ def test_normal_and_loguniform(bws):
     assert e.min() >= -3.0001 and e.max() <= 3.0001 and abs(np.mean(y > 0) - 0.5) < 0.01
     u = bws.gen_f32(2, 100000, "uniform", -1.0, 1.0)
     assert u.min() >= -1 and u.max() < 1 and abs(u.mean()) < 0.01
+
+
+def test_split_workspace_size(bws):
+    """The MFMA density split asks for the block flags on top of the rounded B."""
+    s = bws._Bitmap(300, 128, 4, None, None, None)
+    L = bws.lib()
+    plain = L.bws_spmm_workspace_size(C.byref(s), 64, bws.PRECISION["bf16"], None)
+    split = L.bws_spmm_workspace_size(C.byref(s), 64, bws.PRECISION["bf16"],
+                                      C.byref(bws.make_plan(mfma_min_bits=8)))
+    assert split >= plain + 256  # the block flags on top of the rounded B
