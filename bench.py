@@ -748,6 +748,14 @@ def _comparators(bws, A, W, Bd, M, N, K, nnz, dev):
         if check("dense_bf16", torch.matmul(Ab, Bb).float(), 1e-2):
             spin_up(lambda: torch.matmul(Ab, Bb), 0.1)
             out["dense_bf16_ms"] = round(time_loop(lambda: torch.matmul(Ab, Bb), 20, 2)[1], 4)
+        # the same BF16 operands with C written in FP32, as ours is
+        try:
+            f = lambda: torch.mm(Ab, Bb, out_dtype=torch.float32)
+            if check("dense_bf16_f32out", f(), 1e-2):
+                spin_up(f, 0.1)
+                out["dense_bf16_f32out_ms"] = round(time_loop(f, 20, 2)[1], 4)
+        except Exception as e:
+            out["dense_bf16_f32out_error"] = str(e)[:200]
         del Ab, Bb
     except Exception as e:
         out["dense_error"] = str(e)
