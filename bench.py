@@ -35,6 +35,18 @@ ADD_RATE_UNPACKED = ADD_RATE_PACKED / 2
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "spmm_{}_4096_clustered_traffic.json")
 
 
+def sources_sha256(paths):
+    """SHA-256 over the named source files (paths relative to the repository
+    root, in order); the counter files record it for the kernel they measured,
+    so that a reading taken before the kernel changed is not reported."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in paths:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest() if paths else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -226,13 +238,19 @@ def main():
     kname = {"fp32_rowclusters": "k_spmm_stream<true>",
              "fp32": "k_spmm_stream<false>"}.get(path)
     traffic = None
+    traffic_note = None
     tfile = PROFILE_TRAFFIC.format(path)
     if (os.path.exists(tfile) and not args.iid and M == N == K == 4096 and args.p == 0.03125
             and args.flip == 0.02):
         try:
             tj = json.load(open(tfile))
-            # only a counter reading of the kernel that runs now
-            if kname is None or tj.get("kernel") == kname:
+            # only a counter reading of the kernel that runs now, taken from
+            # the kernel sources as they are now
+            if kname is not None and tj.get("kernel") != kname:
+                traffic_note = f"counters are for {tj.get('kernel')}"
+            elif tj.get("sources_sha256") != sources_sha256(tj.get("sources", [])):
+                traffic_note = "stale: the kernel sources changed since the counter run"
+            else:
                 traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -244,6 +262,7 @@ def main():
         roofline = {"kernel": kname,
                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_note": traffic_note,
                     "kernel_ms": round(ms_kernel, 5), "algorithmic_bytes": alg_bytes,
                     "ceiling": {"hbm_us": round(t_hbm_us, 2), "fp32_add_us": round(t_add_us, 2),
                                 "fp32_add_unpacked_us": round(t_add_unpacked_us, 2),
@@ -262,6 +281,7 @@ def main():
                     "mfma_path": "sparse (v_smfmac, dense-equivalent flops)"
                     if mplan is not None else "dense",
                     "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFS, 4), "traffic": traffic,
+                    "traffic_note": traffic_note,
                     "kernel_ms": round(ms_kernel, 5), "mfma_flops": mfma_flops,
                     "hbm_achieved_GBps": round(achieved, 1), "algorithmic_bytes": alg_bytes,
                     "ceiling": {"hbm_us": round(t_hbm_us, 2), "mfma_us": round(t_mfma_us, 2),
