@@ -9,11 +9,11 @@
 // consecutive set bits of one row (a row with an odd count ends with a single
 // bit) and belongs to a quad of lanes: lane p of the quad reads 16-byte chunks
 // 4i + p (i = 0..3) of the dC row once and of both B rows, i.e. 12 LDS reads
-// for two dot products instead of 16.  Chunk c of staged row r sits at slot
-// 4 ((c >> 2) ^ (r & 3)) + ((c & 3) ^ ((r >> 2) & 3)): a quad fills one
-// aligned group of four slots, the dC reads of a lane take fixed offsets from
-// one base (its four groups in order), and the four quads of a 16-lane LDS
-// pass meet in a group only when their rows agree in bits 0-1.
+// for two dot products instead of 16.  Staged rows are 256 bytes, one full
+// span of the LDS banks, in chunk order.  At step g the quad in place j of its
+// 16-lane LDS pass (j = (lane >> 2) & 3) reads chunk group i = (g + j) & 3 of
+// its rows, so the four quads of a pass read the four different 64-byte
+// groups, all banks once, whatever rows their items have.
 //
 // Each lane keeps four partial sums per bit (components x, y, z, w of its
 // chunks) over N in ascending stages of 64; at the end they are added as
@@ -40,10 +40,6 @@ constexpr int G = 2;                  // set bits of one row per work item
 constexpr int SLOTS = 8;              // work items per quad of lanes per round
 constexpr int SPARSE = 64;            // tiles with fewer set bits are left to the row kernel
 
-__device__ __forceinline__ uint32_t chunk_slot(int r, int c) {
-  return (uint32_t)(4 * ((c >> 2) ^ (r & 3)) + ((c & 3) ^ ((r >> 2) & 3)));
-}
-
 __device__ __forceinline__ void fma4(float4& acc, const float4& a, const float4& b) {
   acc.x = fmaf(a.x, b.x, acc.x);
   acc.y = fmaf(a.y, b.y, acc.y);
@@ -65,14 +61,13 @@ __device__ __forceinline__ void tile_rounds(const float* __restrict__ dC, int64_
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // copy sources: instruction g of a stage fills staged rows 4g .. 4g+3; lane
-  // l writes slot l % 16 of row 4g + l / 16, which holds chunk
-  // chunk_slot(row, l % 16) (the mapping is its own inverse)
+  // l writes chunk l % 16 of row 4g + l / 16
   const float* dsrc[NCW];
   const float* bsrc[NCW];
 #pragma unroll
   for (int t = 0; t < NCW; ++t) {
     const int row = 4 * (wave * NCW + t) + lane / 16;
-    const int c = (int)chunk_slot(row, lane % 16);
+    const int c = lane % 16;
     int64_t rd = r0 + row, rk = k0 + row;
     if (rd > M - 1) rd = M - 1;
     if (rk > K - 1) rk = K - 1;
@@ -93,16 +88,17 @@ __device__ __forceinline__ void tile_rounds(const float* __restrict__ dC, int64_
 
   const int ph = lane & 3;                  // this lane's place in its quad
   const int quad = wave * 16 + (lane >> 2);  // 0 .. 63
+  // byte offset within a staged row of the chunk this lane reads at step g
+  uint32_t coff[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) coff[g] = (uint32_t)(64 * ((g + (lane >> 2)) & 3) + 16 * ph);
   const uint8_t* sDp = &sD[0][0];
   const uint8_t* sBp = &sB[0][0];
   for (int q = 0; q < n_items; q += ROUND) {
     // Item i of the tile: set bits G p .. G p + G-1 (in column order) of one
-    // row.  Per item: the dC row's read offsets and, per bit, the B row's.
-    // With x = r & 3 and y = (r >> 2) & 3, chunk 4i + ph of staged row r sits
-    // at slot 4 (i ^ x) + (ph ^ y): reading the dC row's slot groups g = 0..3
-    // in order takes its chunks i = g ^ x_d, whose B chunks are in groups
-    // g ^ x_d ^ x_b.
-    uint32_t offD[SLOTS], offB[SLOTS][G][4];
+    // row.  Per item and step: the dC row's read offset and, per bit, the B
+    // row's (the same chunk of each).
+    uint32_t offD[SLOTS][4], offB[SLOTS][G][4];
     int64_t out[SLOTS];
     int nb[SLOTS];  // set bits of the item (0 .. G)
 #pragma unroll
@@ -128,16 +124,15 @@ __device__ __forceinline__ void tile_rounds(const float* __restrict__ dC, int64_
         out[s] = sBase[rr] + first;
         for (int x = 0; x < first; ++x) m &= ~(0x8000000000000000ull >> __clzll(m));
       }
-      offD[s] = (uint32_t)(rr * ROWB + 16 * (ph ^ ((rr >> 2) & 3)));
+#pragma unroll
+      for (int g = 0; g < 4; ++g) offD[s][g] = (uint32_t)(rr * ROWB) + coff[g];
 #pragma unroll
       for (int b = 0; b < G; ++b) {
         // column of the item's bit b (row 0 when the item has fewer bits)
         const int kk = b < nb[s] ? __clzll(m) : 0;
         if (b < nb[s]) m &= ~(0x8000000000000000ull >> kk);
-        const int z = (rr ^ kk) & 3;
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
-          offB[s][b][g] = (uint32_t)(kk * ROWB + 16 * (ph ^ ((kk >> 2) & 3)) + 64 * (g ^ z));
+        for (int g = 0; g < 4; ++g) offB[s][b][g] = (uint32_t)(kk * ROWB) + coff[g];
       }
     }
     float4 acc[SLOTS][G];
@@ -158,7 +153,7 @@ __device__ __forceinline__ void tile_rounds(const float* __restrict__ dC, int64_
         if (s < used) {  // some quad of this wave has slot s
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
-            const float4 a = *reinterpret_cast<const float4*>(sDp + buf + offD[s] + 64 * g);
+            const float4 a = *reinterpret_cast<const float4*>(sDp + buf + offD[s][g]);
 #pragma unroll
             for (int b = 0; b < G; ++b) {
               const float4 v = *reinterpret_cast<const float4*>(sBp + buf + offB[s][b][g]);
