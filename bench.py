@@ -158,6 +158,10 @@ def main():
                     help="16-bit precisions: dense MFMA (auto) or the sparse MFMA")
     ap.add_argument("--fp32-path", default="auto", choices=["auto", "clustered", "plain"],
                     help="FP32 kernel: the row-clustered product, the plain one, or by the plan")
+    ap.add_argument("--leg", default=None,
+                    choices=["density_sweep", "value_mask", "int8_bitplanes", "mfma_split",
+                             "grouped", "comparators"],
+                    help="extras: run only this leg (for profiling one workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -374,6 +378,14 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     import numpy as np
     import torch
     ex = {}
+    legs = {"density_sweep": lambda: _density_sweep(bws, dev, M, N, K, args),
+            "value_mask": lambda: _value_mask_workload(bws, dev),
+            "int8_bitplanes": lambda: _bitplanes_workload(bws, dev, Bd, M, N, K),
+            "mfma_split": lambda: _split_workload(bws, dev, Bd, M, N, K),
+            "grouped": lambda: _grouped_workload(bws, dev),
+            "comparators": lambda: _comparators(bws, A, W, Bd, M, N, K, nnz, dev)}
+    if args.leg:
+        return {args.leg: legs[args.leg]()}
     def tl(f, k=20, w=3):
         spin_up(f, 0.1)   # steady clocks before every comparison
         return round(time_loop(f, k, w)[1], 4)
@@ -459,12 +471,8 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     del AT, dC, dB, ref, refT
 
     if not args.quick:
-        ex["density_sweep"] = _density_sweep(bws, dev, M, N, K, args)
-        ex["value_mask"] = _value_mask_workload(bws, dev)
-        ex["int8_bitplanes"] = _bitplanes_workload(bws, dev, Bd, M, N, K)
-        ex["mfma_split"] = _split_workload(bws, dev, Bd, M, N, K)
-        ex["grouped"] = _grouped_workload(bws, dev)
-        ex["comparators"] = _comparators(bws, A, W, Bd, M, N, K, nnz, dev)
+        for name, leg in legs.items():
+            ex[name] = leg()
     return ex
 
 
