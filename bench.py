@@ -378,14 +378,14 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     import numpy as np
     import torch
     ex = {}
-    legs = {"density_sweep": lambda: _density_sweep(bws, dev, M, N, K, args),
+    leg_fns = {"density_sweep": lambda: _density_sweep(bws, dev, M, N, K, args),
             "value_mask": lambda: _value_mask_workload(bws, dev),
             "int8_bitplanes": lambda: _bitplanes_workload(bws, dev, Bd, M, N, K),
             "mfma_split": lambda: _split_workload(bws, dev, Bd, M, N, K),
             "grouped": lambda: _grouped_workload(bws, dev),
             "comparators": lambda: _comparators(bws, A, W, Bd, M, N, K, nnz, dev)}
     if args.leg:
-        return {args.leg: legs[args.leg]()}
+        return {args.leg: leg_fns[args.leg]()}
     def tl(f, k=20, w=3):
         spin_up(f, 0.1)   # steady clocks before every comparison
         return round(time_loop(f, k, w)[1], 4)
@@ -471,7 +471,7 @@ def run_extras(bws, A, W, Bd, M, N, K, nnz, dev, args, clusters=None):
     del AT, dC, dB, ref, refT
 
     if not args.quick:
-        for name, leg in legs.items():
+        for name, leg in leg_fns.items():
             ex[name] = leg()
     return ex
 
