@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from _helpers import to_dev
+from _helpers import make_problem, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -65,3 +65,23 @@ def test_split_alpha_beta_strides(oracle, bws, dev, rng):
     assert np.all(np.abs(got[:, 8:8 + N] - expect) <= 1e-5 * (0.5 * S + 2 * np.abs(C0[:, 8:8 + N])) + 1e-6)
     assert np.array_equal(got[:, :8], C0[:, :8]) and np.array_equal(got[:, 8 + N:], C0[:, 8 + N:])
 
+
+
+@pytest.mark.parametrize("case", ["values", "n_not_mult_8"])
+def test_split_not_applicable(oracle, bws, dev, case):
+    """The split needs a 0/1 A and N % 8 == 0; otherwise the plan field is
+    ignored and the product is the unsplit one (value-carrying A: its values
+    rounded to BF16 too, hence the 1e-2 bound against FP64)."""
+    M, K = 512, 384
+    N = 136 if case == "values" else 131
+    A, W, rp, vals, B, Bd = make_problem(bws, oracle, dev, M, K, N, 0.05, seed=21,
+                                         values=case == "values")
+    Bh = Bd.to(torch.bfloat16)
+    Bexact = Bh.float().cpu().numpy()
+    got = bws.spmm(A, Bh, "bf16", plan=bws.make_plan(mfma_min_bits=40)).cpu().numpy()
+    ref = bws.spmm(A, Bh, "bf16").cpu().numpy()
+    R = oracle.spmm_f64(W, K, Bexact, rp, vals)
+    S = oracle.spmm_f64(W, K, Bexact, rp, vals, absolute=True)
+    bound = 1e-2 if case == "values" else 1e-5
+    assert np.all(np.abs(got - R) <= bound * S + 1e-30)
+    assert np.all(np.abs(got - ref) <= 2e-5 * S + 1e-30)
