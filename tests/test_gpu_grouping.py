@@ -63,6 +63,12 @@ def test_grouping_known_answers_gpu(oracle, bws, dev, case):
     (300, 32, 7, 1, 0.5, True),
     (5000, 33, 4096, 3, 0.01, False),
     (128, 96, 65536, 2, 0.1, False),
+    # every compiled group count (1-4 with the masks in scalar registers, more
+    # through the generic path), 0/1 and value-carrying, tiles of 1-3 load groups
+    (1000, 100, 64, 4, 0.02, False),
+    (3000, 64, 3000, 3, 0.03, True),
+    (777, 70, 100, 8, 0.3, True),
+    (2500, 40, 2500, 1, 0.02, False),
 ])
 def test_group_rows_parity(oracle, bws, dev, rng, rows, cols, T, G, p, values):
     D = rng.standard_normal((rows, cols)).astype(np.float32)
