@@ -193,6 +193,7 @@ bws_status bws_spmm(const bws_bitmap* A, const void* B, int32_t b_dtype, int64_t
   const bool subset = plan && plan->kernel == 1 && plan->rows_per_wg == 0 &&
                       plan->k_stage == kSubsetKStage;
   if (plan && (plan->kernel < 0 || plan->kernel > 2 || plan->rows_per_wg < 0 ||
+               plan->mfma_min_bits < 0 ||
                (plan->kernel != 2 && plan->rows_per_wg != 0 &&
                 !valu_cfg_valid(plan_valu_cfg(plan, A->rows, N, A->values != nullptr))) ||
                (plan->kernel == 2 && !mfma_tile_valid(plan->rows_per_wg, plan->cols_per_wg))))

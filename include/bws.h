@@ -333,7 +333,8 @@ bws_status bws_cluster_rows(const bws_bitmap* A, int32_t max_clusters, int32_t* 
  *                 others on the direct-to-LDS MFMA kernel; needs N % 8 == 0,
  *                 16-byte aligned rows of B (or of B rounded once, for FP32
  *                 B) and the workspace bws_spmm_workspace_size reports (the
- *                 block flags and room to round an FP32 B); otherwise ignored
+ *                 block flags and room to round an FP32 B); otherwise ignored;
+ *                 negative: BWS_EINVAL
  *   cols_per_wg   MFMA path: columns of C per workgroup (0 = auto, 128 or 256;
  *                 256 x 256 only with the direct-to-LDS staging of 16-bit B,
  *                 else it runs as 128 x 256; BF16X2 uses 128)

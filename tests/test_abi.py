@@ -118,6 +118,9 @@ def test_spmm_validation(bws):
     plan = bws.make_plan(rows_per_wg=100)
     assert L.bws_spmm(C.byref(a), FAKE, 0, 64, 64, FAKE, 64, 1.0, 0.0, 0, C.byref(plan), None, 0,
                       None) == bws.BWS_EINVAL
+    plan = bws.make_plan(mfma_min_bits=-1)   # 0 = off; no negative thresholds
+    assert L.bws_spmm(C.byref(a), FAKE, 1, 64, 64, FAKE, 64, 1.0, 0.0, 1, C.byref(plan), None, 0,
+                      None) == bws.BWS_EINVAL
 
 
 def test_sddmm_and_groups_validation(bws):
