@@ -122,6 +122,17 @@ def cpu_baseline(bws, W, B, K, N, target_s, threads):
            "sample": f"first {rows} of {M} rows of the same A (dense 0/1 form, nnz {nnz}) x the "
                      f"full {K}x{N} B: check() loop i,j,p in FP32, OpenMP over rows, "
                      f"{dt:.2f} s"}
+    # for information: the oracle's sparse-aware FP64 product (one thread,
+    # bitmap-driven, double accumulation) on the same row sample
+    import numpy as np
+    import oracle
+    Ws = np.ascontiguousarray(W[:rows])
+    t0 = time.perf_counter()
+    oracle.spmm_f64(Ws, K, B)
+    dtf = time.perf_counter() - t0
+    out["fp64_oracle_sparse"] = {"value": round(2.0 * nnz * N / dtf / 1e12, 6), "cores": 1,
+                                 "sample": f"the same {rows} rows, {dtf:.2f} s",
+                                 "note": "information only: the checker's FP64 product"}
     rows1, nnz1, dt1 = _check_rate(W, B, K, N, target_s / 2, 1, 1)
     out["single_thread"] = {"value": round(2.0 * nnz1 * N / dt1 / 1e12, 6), "cores": 1,
                             "sample": f"first {rows1} rows, nnz {nnz1}, {dt1:.2f} s"}
