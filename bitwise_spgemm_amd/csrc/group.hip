@@ -88,13 +88,18 @@ k_group_rows(const uint32_t* __restrict__ W, int64_t rows, int64_t cols, int64_t
              float* __restrict__ group_vals, int32_t* __restrict__ member_row,
              int32_t* __restrict__ spill_cnt, int32_t* __restrict__ spill_nnz,
              int32_t* __restrict__ spill_table, int32_t* __restrict__ spill_rev,
-             unsigned long long* __restrict__ grouped_nnz) {
+             unsigned long long* __restrict__ grouped_nnz, int64_t per_xcd) {
   // the words of a group of chunks, then each row's result packed as
   // (slot + 1) << 8 | (group + 1) (group -1..31, slot -1..65535)
   __shared__ uint32_t s_word[kGroupPF * 64];
   __shared__ uint32_t s_res[kGroupPF * 64];
   const int lane = lane_id();
-  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+  // XCD-contiguous tile ranges: the tiles of neighbouring words of the same
+  // rows share an XCD, so the 128-byte lines of group_id they fill together
+  // are merged in one L2 before they are written
+  for (int64_t b = blockIdx.x; b < 8 * per_xcd; b += gridDim.x) {
+    const int64_t t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= tiles) continue;
     const int64_t rt = t / wpr, w = t - rt * wpr;
     const int64_t row0 = rt * T;
     const int64_t rem = rows - row0;
@@ -343,12 +348,15 @@ bws_status bws_group_rows(const bws_bitmap* A, int32_t tile_rows, int32_t max_gr
   if (tiles == 0) return launch_check();
   const int64_t wpr = words_for(A->cols);
   auto* gn = reinterpret_cast<unsigned long long*>(out->grouped_nnz);
+  const int64_t per_xcd = ceil_div(tiles, 8);
+  // a multiple of 8 (each workgroup keeps to one XCD's range), at most 65536
+  const int grid = (int)(8 * per_xcd < 65536 ? 8 * per_xcd : 65536);
 #define BWS_GROUP(HV, GC)                                                                   \
-  hipLaunchKernelGGL((k_group_rows<HV, GC>), dim3(grid_for(tiles, 1)), dim3(64), 0, stream,       \
+  hipLaunchKernelGGL((k_group_rows<HV, GC>), dim3(grid), dim3(64), 0, stream,                     \
                      A->words, A->rows, A->cols, wpr, A->ldw, A->row_ptr, A->values,            \
                      (int)tile_rows, (int)max_groups, tiles, out->group_id, out->group_mask,     \
                      out->group_vals, out->member_row, out->spill_cnt, out->spill_nnz,          \
-                     out->spill_table, out->spill_rev, gn)
+                     out->spill_table, out->spill_rev, gn, per_xcd)
 #define BWS_GROUP_G(HV)              \
   switch (max_groups) {              \
     case 1: BWS_GROUP(HV, 1); break; \
